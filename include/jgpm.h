@@ -1,0 +1,188 @@
+/*
+ * jgpm.h — C ABI of the MI355X (gfx950) graph pattern counting library.
+ *
+ * Plain C types only: pointers, sizes and integer status codes.  Every entry
+ * point returns JGPM_OK (0) or a negative JGPM_E_* status; nothing in the
+ * library calls exit() or abort().  jgpm_strerror() maps a status to text.
+ *
+ * What each entry point replaces in the reference (AnySparse/Jupiter):
+ *   jgpm_init / jgpm_shutdown      cudaSetDevice + nvshmemx_init_attr in
+ *                                  SubgraphSolver (src/jupiter_gpm.cu:39-58)
+ *   jgpm_graph_upload / _load      Graph(prefix, use_dag) (common/graph.cc:8-124)
+ *                                  + NvsGraphGPU::init (include/graph_gpu_nvs.h:258-304)
+ *   jgpm_count                     the round loop + name->kernel dispatch + count
+ *                                  readback (src/jupiter_gpm.cu:356-908)
+ *   jgpm_count_batch               no reference counterpart (one query per run there)
+ *   jgpm_serve_*                   no reference counterpart (continuous batching)
+ *   jgpm_gen_* / jgpm_csr_*        host graph construction used by the CLI, tests
+ *                                  and benchmarks; orientation follows
+ *                                  Graph::orientation (common/graph.cc:210-401)
+ */
+#ifndef JGPM_H
+#define JGPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define JGPM_OK              0
+#define JGPM_E_INVALID      -1   /* bad argument (null pointer, bad range, bad flags) */
+#define JGPM_E_PATTERN      -2   /* unknown pattern id (reference: assert(0), src/jupiter_gpm.cu:428,872) */
+#define JGPM_E_NOMEM        -3   /* device or host allocation failed */
+#define JGPM_E_HIP          -4   /* a HIP runtime call failed */
+#define JGPM_E_IO           -5   /* file missing / short read / bad meta file */
+#define JGPM_E_BUDGET       -6   /* request exceeds the HBM budget of the serving loop */
+#define JGPM_E_GRAPHKIND    -7   /* pattern needs the symmetric CSR but the graph was uploaded as DAG-only */
+#define JGPM_E_BUSY         -8   /* serving queue full, retry */
+#define JGPM_E_NOTREADY     -9   /* serving result not ready yet */
+#define JGPM_E_STATE       -10   /* call not valid in the current state */
+#define JGPM_E_NODEVICE    -11   /* no HIP device available */
+
+/* ---- pattern ids ------------------------------------------------------- *
+ * P1..P16 keep the reference's names and meanings (the kernels of
+ * src/message_passing_gpu_kernels.cuh:104-1537).  Each equals the
+ * non-induced subgraph count of a fixed pattern graph H:
+ *   P1 4-cycle, P2 diamond, P3 triangle with a 2-path tail, P4 house,
+ *   P5 K4 plus a vertex on an edge, P6 K_{2,3} plus an edge, P7 gem,
+ *   P8 wheel W4, P9 K5 minus an edge, P10 5-fan, P11 3-sun,
+ *   P12 the (6 vertex, 11 edge) graph of P12_kernel, P13..P16 K4..K7.
+ * Extension ids (no reference P-id):
+ *   JGPM_TC        triangles (reference P0_kernel / total[1])
+ *   JGPM_STAR3     non-induced 3-stars, sum_v C(d_v,3)
+ *   JGPM_TAILTRI   non-induced tailed triangles, sum_v t_v (d_v - 2)
+ *   JGPM_I_*       vertex-induced 4-vertex motifs obtained from the
+ *                  non-induced counts by the exact inverse conversion matrix
+ */
+enum {
+  JGPM_TC = 0,
+  JGPM_P1 = 1, JGPM_P2, JGPM_P3, JGPM_P4, JGPM_P5, JGPM_P6, JGPM_P7, JGPM_P8,
+  JGPM_P9, JGPM_P10, JGPM_P11, JGPM_P12, JGPM_P13, JGPM_P14, JGPM_P15, JGPM_P16,
+  JGPM_STAR3 = 17,
+  JGPM_TAILTRI = 18,
+  JGPM_I_C4 = 20,      /* induced 4-cycle            */
+  JGPM_I_DIAMOND = 21, /* induced diamond            */
+  JGPM_I_TAILTRI = 22, /* induced tailed triangle    */
+  JGPM_I_STAR3 = 23,   /* induced 3-star             */
+  JGPM_I_K4 = 24,      /* induced K4 (== P13)        */
+  JGPM_NUM_PATTERNS = 25
+};
+
+/* graph upload flags */
+#define JGPM_SYM 1u   /* the CSR is symmetric (undirected); the library derives the degree DAG */
+#define JGPM_DAG 2u   /* the CSR is already the degree-oriented DAG; only DAG patterns allowed */
+
+typedef struct jgpm_ctx_s*   jgpm_ctx_t;
+typedef struct jgpm_graph_s* jgpm_graph_t;
+typedef struct jgpm_csr_s*   jgpm_csr_t;
+typedef struct jgpm_serve_s* jgpm_serve_t;
+
+/* ---- context ----------------------------------------------------------- */
+/* Opens HIP device `dev`, creates the context's stream and work buffers. */
+int  jgpm_init(int dev, jgpm_ctx_t* out);
+int  jgpm_shutdown(jgpm_ctx_t ctx);
+const char* jgpm_strerror(int status);
+/* Device properties: CU count, LDS bytes per workgroup, HBM bytes. */
+int  jgpm_device_info(jgpm_ctx_t ctx, int* num_cu, int* lds_bytes, int64_t* hbm_bytes,
+                      char* name, int name_len);
+
+/* ---- graphs ------------------------------------------------------------ */
+/* Copies a host CSR (int64 rowptr[nv+1], int32 colidx[nnz], ascending lists)
+ * to the device.  The caller keeps ownership of the host arrays. */
+int  jgpm_graph_upload(jgpm_ctx_t ctx, const int64_t* rowptr, const int32_t* colidx,
+                       int32_t nv, int64_t nnz, uint32_t flags, jgpm_graph_t* out);
+/* Loads <prefix>.meta.txt / .vertex.bin / .edge.bin (format of
+ * Graph::Graph and Graph::dump_binary, common/graph.cc:26-46,727-753).
+ * dag != 0 keeps only the degree DAG (as Graph(prefix, use_dag=true)). */
+int  jgpm_graph_load(jgpm_ctx_t ctx, const char* prefix, int dag, jgpm_graph_t* out);
+int  jgpm_graph_free(jgpm_graph_t g);
+/* Sizes of the resident graph: vertices, symmetric nnz, DAG nnz, max degrees. */
+int  jgpm_graph_info(jgpm_graph_t g, int32_t* nv, int64_t* sym_nnz, int64_t* dag_nnz,
+                     int32_t* sym_max_deg, int32_t* dag_max_deg);
+/* Device bytes held by the graph handle. */
+int  jgpm_graph_bytes(jgpm_graph_t g, int64_t* bytes);
+
+/* ---- counting ---------------------------------------------------------- */
+/* Count pattern `pattern` over the roots [root_begin, root_end) — the
+ * source vertices v0 of the matched edges, as the reference's per-GPU edge
+ * ranges (src/jupiter_gpm.cu:153-162,373-375).  root_end = -1 means nv.
+ * Blocks until the count is on the host.  kernel_ms (may be NULL) receives
+ * the device time between the first and last kernel of the query. */
+int  jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
+                uint64_t* count, double* kernel_ms);
+/* Algorithmic bytes of the reference's canonical loop nest for the same
+ * query: every neighbour-list fetch costs 16 + 4*d(v) bytes.  Computed on
+ * the device.  Defined for JGPM_TC and P13..P16. */
+int  jgpm_work_bytes(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
+                     uint64_t* bytes);
+
+/* Tuning knobs (autotuner output); 0 keeps the current value. */
+typedef struct {
+  int32_t small_max_deg;    /* roots with out-degree <= this use one wave each (<= 64)   */
+  int32_t lds_max_deg;      /* roots up to this out-degree keep their bitmap in LDS      */
+  int32_t gallop_ratio;     /* neighbour list longer than ratio*d(root) -> binary search  */
+  int32_t waves_per_block;  /* waves per workgroup for the wave-per-root kernel           */
+  int32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * num_cu              */
+} jgpm_tuning;
+int  jgpm_get_tuning(jgpm_ctx_t ctx, jgpm_tuning* t);
+int  jgpm_set_tuning(jgpm_ctx_t ctx, const jgpm_tuning* t);
+
+/* ---- batching: many (graph, pattern, range) queries in one launch ------ */
+typedef struct {
+  jgpm_graph_t graph;
+  int32_t pattern;
+  int32_t _pad;
+  int64_t root_begin;
+  int64_t root_end;      /* -1 = nv */
+} jgpm_query;
+int  jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* queries, int n, uint64_t* counts,
+                      double* kernel_ms);
+
+/* ---- serving: continuous batching over device-resident graphs ---------- */
+typedef struct {
+  int64_t hbm_budget_bytes;  /* admission budget for per-query intermediates   */
+  int32_t max_inflight;      /* queries admitted to the device at once         */
+  int32_t slice_roots;       /* roots per scheduling slice (fairness)          */
+} jgpm_serve_config;
+int  jgpm_serve_start(jgpm_ctx_t ctx, const jgpm_serve_config* cfg, jgpm_serve_t* out);
+/* Enqueue a query; returns a ticket.  Thread-safe. */
+int  jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, int64_t root_begin,
+                       int64_t root_end, uint64_t* ticket);
+/* Non-blocking: JGPM_E_NOTREADY until the ticket's count is available.
+ * latency_us (may be NULL): submit -> result-visible time. */
+int  jgpm_serve_poll(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us);
+int  jgpm_serve_wait(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us);
+int  jgpm_serve_stop(jgpm_serve_t s);
+
+/* ---- host CSR construction (generators, orientation, files) ------------ */
+/* Seeded R-MAT: 2^scale vertices, edge_factor * 2^scale samples, quadrant
+ * probabilities a, b, c (d = 1-a-b-c); self loops dropped, symmetrised,
+ * deduplicated, ascending lists.  Bit-reproducible for a given seed. */
+int  jgpm_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                   jgpm_csr_t* out);
+/* Chung-Lu power law: weights (i + i0)^(-1/(gamma-1)) scaled to target
+ * avg edges and max weight; alias-method endpoint sampling. */
+int  jgpm_gen_chung_lu(int32_t n, int64_t target_edges, double gamma, double max_weight,
+                       uint64_t seed, jgpm_csr_t* out);
+/* Erdos-Renyi with the given mean degree (uniform pair sampling). */
+int  jgpm_gen_er(int32_t n, double mean_degree, uint64_t seed, jgpm_csr_t* out);
+/* Barabasi-Albert preferential attachment, m edges per new vertex. */
+int  jgpm_gen_ba(int32_t n, int32_t m, uint64_t seed, jgpm_csr_t* out);
+/* Build a symmetric CSR from an arbitrary undirected edge list. */
+int  jgpm_csr_from_edges(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
+                         jgpm_csr_t* out);
+/* Degree-ordered DAG of a symmetric CSR: keep u->v iff d(v) > d(u) or
+ * (d(v) == d(u) and v > u) (common/graph.cc:229-230). */
+int  jgpm_csr_orient(jgpm_csr_t sym, jgpm_csr_t* dag);
+int  jgpm_csr_info(jgpm_csr_t c, int32_t* nv, int64_t* nnz, int32_t* max_deg);
+int  jgpm_csr_data(jgpm_csr_t c, const int64_t** rowptr, const int32_t** colidx);
+int  jgpm_csr_write(jgpm_csr_t c, const char* prefix);
+int  jgpm_csr_read(const char* prefix, jgpm_csr_t* out);
+int  jgpm_csr_free(jgpm_csr_t c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JGPM_H */

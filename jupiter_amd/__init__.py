@@ -1,0 +1,338 @@
+"""jupiter_amd — MI355X-native graph pattern counting (Python host mirror).
+
+Thin ctypes layer over the C ABI of ``include/jgpm.h`` (``libjgpm.so``, HIP
+kernels for gfx950).  It mirrors the reference's operator surface:
+
+* ``Graph``: a resident CSR graph (reference ``Graph(prefix, use_dag)``,
+  common/graph.cc:8-124 plus ``NvsGraphGPU::init``, include/graph_gpu_nvs.h:258-304);
+* ``Graph.count("P13", roots=(b, e))``: one pattern query over a root range
+  (the reference's ``SubgraphSolver`` dispatch, src/jupiter_gpm.cu:391-908);
+* pattern names ``P1``..``P16`` as in the reference, plus ``TC`` and the
+  induced 4-vertex motif ids.
+
+There is no CPU fallback: if ``libjgpm.so`` is missing or no GPU is present the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libjgpm.so")
+
+PATTERNS = {
+    "TC": 0,
+    **{f"P{i}": i for i in range(1, 17)},
+    "STAR3": 17,
+    "TAILTRI": 18,
+    "I_C4": 20,
+    "I_DIAMOND": 21,
+    "I_TAILTRI": 22,
+    "I_STAR3": 23,
+    "I_K4": 24,
+}
+PATTERN_NAMES = {v: k for k, v in PATTERNS.items()}
+DAG_PATTERNS = {0, 13, 14, 15, 16, 24}
+
+SYM = 1
+DAG = 2
+
+STATUS = {
+    0: "ok", -1: "invalid argument", -2: "unknown pattern id", -3: "out of memory", -4: "HIP runtime error",
+    -5: "graph file missing or malformed", -6: "over budget / unsupported size", -7: "needs symmetric graph",
+    -8: "busy", -9: "not ready", -10: "bad state", -11: "no HIP device",
+}
+
+
+class JgpmError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({status})")
+
+
+_lib = None
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("small_max_deg", ctypes.c_int32), ("lds_max_deg", ctypes.c_int32),
+                ("gallop_ratio", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32)]
+
+
+class _Query(ctypes.Structure):
+    _fields_ = [("graph", ctypes.c_void_p), ("pattern", ctypes.c_int32), ("_pad", ctypes.c_int32),
+                ("root_begin", ctypes.c_int64), ("root_end", ctypes.c_int64)]
+
+
+class _ServeCfg(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
+                ("slice_roots", ctypes.c_int32)]
+
+
+def lib():
+    """Load libjgpm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m jupiter_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I32, I64, U64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "jgpm_init": ([ctypes.c_int, PP], ctypes.c_int),
+        "jgpm_shutdown": ([P], ctypes.c_int),
+        "jgpm_strerror": ([ctypes.c_int], ctypes.c_char_p),
+        "jgpm_device_info": ([P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(I64), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        "jgpm_graph_upload": ([P, P, P, I32, I64, ctypes.c_uint32, PP], ctypes.c_int),
+        "jgpm_graph_load": ([P, ctypes.c_char_p, ctypes.c_int, PP], ctypes.c_int),
+        "jgpm_graph_free": ([P], ctypes.c_int),
+        "jgpm_graph_info": ([P, ctypes.POINTER(I32), ctypes.POINTER(I64), ctypes.POINTER(I64),
+                             ctypes.POINTER(I32), ctypes.POINTER(I32)], ctypes.c_int),
+        "jgpm_graph_bytes": ([P, ctypes.POINTER(I64)], ctypes.c_int),
+        "jgpm_count": ([P, ctypes.c_int, I64, I64, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)],
+                       ctypes.c_int),
+        "jgpm_work_bytes": ([P, ctypes.c_int, I64, I64, ctypes.POINTER(U64)], ctypes.c_int),
+        "jgpm_get_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "jgpm_set_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "jgpm_count_batch": ([P, ctypes.POINTER(_Query), ctypes.c_int, ctypes.POINTER(U64),
+                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_serve_start": ([P, ctypes.POINTER(_ServeCfg), PP], ctypes.c_int),
+        "jgpm_serve_submit": ([P, P, ctypes.c_int, I64, I64, ctypes.POINTER(U64)], ctypes.c_int),
+        "jgpm_serve_poll": ([P, U64, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_serve_wait": ([P, U64, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_serve_stop": ([P], ctypes.c_int),
+        "jgpm_gen_rmat": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, U64, PP],
+                          ctypes.c_int),
+        "jgpm_gen_chung_lu": ([I32, I64, ctypes.c_double, ctypes.c_double, U64, PP], ctypes.c_int),
+        "jgpm_gen_er": ([I32, ctypes.c_double, U64, PP], ctypes.c_int),
+        "jgpm_gen_ba": ([I32, I32, U64, PP], ctypes.c_int),
+        "jgpm_csr_from_edges": ([I32, I64, P, P, PP], ctypes.c_int),
+        "jgpm_csr_orient": ([P, PP], ctypes.c_int),
+        "jgpm_csr_info": ([P, ctypes.POINTER(I32), ctypes.POINTER(I64), ctypes.POINTER(I32)], ctypes.c_int),
+        "jgpm_csr_data": ([P, PP, PP], ctypes.c_int),
+        "jgpm_csr_write": ([P, ctypes.c_char_p], ctypes.c_int),
+        "jgpm_csr_read": ([ctypes.c_char_p, PP], ctypes.c_int),
+        "jgpm_csr_free": ([P], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(st, what):
+    if st != 0:
+        raise JgpmError(st, what)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pattern_id(p):
+    if isinstance(p, str):
+        if p not in PATTERNS:
+            raise JgpmError(-2, f"pattern {p!r}")
+        return PATTERNS[p]
+    return int(p)
+
+
+# ---------------------------------------------------------------------------
+# Host CSR
+
+
+class CSR:
+    """Host CSR (int64 rowptr, int32 ascending colidx) held in numpy arrays."""
+
+    def __init__(self, rowptr, col):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+
+    @property
+    def nv(self):
+        return len(self.rowptr) - 1
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    def degrees(self):
+        return np.diff(self.rowptr)
+
+    @staticmethod
+    def _from_handle(h):
+        L = lib()
+        nv, nnz, md = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+        _check(L.jgpm_csr_info(h, ctypes.byref(nv), ctypes.byref(nnz), ctypes.byref(md)), "csr_info")
+        rp, ci = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(L.jgpm_csr_data(h, ctypes.byref(rp), ctypes.byref(ci)), "csr_data")
+        r = np.ctypeslib.as_array(ctypes.cast(rp, ctypes.POINTER(ctypes.c_int64)), (nv.value + 1,)).copy()
+        if nnz.value:
+            c = np.ctypeslib.as_array(ctypes.cast(ci, ctypes.POINTER(ctypes.c_int32)), (nnz.value,)).copy()
+        else:
+            c = np.zeros(0, dtype=np.int32)
+        L.jgpm_csr_free(h)
+        return CSR(r, c)
+
+    def _handle(self):
+        """Temporary library-side copy (through the edge list) of this CSR."""
+        src = np.repeat(np.arange(self.nv, dtype=np.int32), np.diff(self.rowptr))
+        h = ctypes.c_void_p()
+        _check(lib().jgpm_csr_from_edges(self.nv, self.nnz, _ptr(src), _ptr(self.col), ctypes.byref(h)),
+               "csr_from_edges")
+        return h
+
+    def orient(self):
+        """Degree-ordered DAG (common/graph.cc:229-230)."""
+        h = self._handle()
+        d = ctypes.c_void_p()
+        st = lib().jgpm_csr_orient(h, ctypes.byref(d))
+        lib().jgpm_csr_free(h)
+        _check(st, "csr_orient")
+        return CSR._from_handle(d)
+
+    def write(self, prefix):
+        h = self._handle()
+        st = lib().jgpm_csr_write(h, prefix.encode())
+        lib().jgpm_csr_free(h)
+        _check(st, "csr_write")
+
+    @staticmethod
+    def read(prefix):
+        h = ctypes.c_void_p()
+        _check(lib().jgpm_csr_read(prefix.encode(), ctypes.byref(h)), "csr_read")
+        return CSR._from_handle(h)
+
+    @staticmethod
+    def from_edges(nv, src, dst):
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().jgpm_csr_from_edges(nv, len(src), _ptr(src), _ptr(dst), ctypes.byref(h)), "csr_from_edges")
+        return CSR._from_handle(h)
+
+
+def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+    h = ctypes.c_void_p()
+    _check(lib().jgpm_gen_rmat(scale, edge_factor, a, b, c, seed, ctypes.byref(h)), "gen_rmat")
+    return CSR._from_handle(h)
+
+
+def chung_lu(n, target_edges, gamma=2.3, max_weight=800.0, seed=1):
+    h = ctypes.c_void_p()
+    _check(lib().jgpm_gen_chung_lu(n, target_edges, gamma, max_weight, seed, ctypes.byref(h)), "gen_chung_lu")
+    return CSR._from_handle(h)
+
+
+def erdos_renyi(n, mean_degree, seed=1):
+    h = ctypes.c_void_p()
+    _check(lib().jgpm_gen_er(n, mean_degree, seed, ctypes.byref(h)), "gen_er")
+    return CSR._from_handle(h)
+
+
+def barabasi_albert(n, m, seed=1):
+    h = ctypes.c_void_p()
+    _check(lib().jgpm_gen_ba(n, m, seed, ctypes.byref(h)), "gen_ba")
+    return CSR._from_handle(h)
+
+
+# ---------------------------------------------------------------------------
+# Device context and resident graphs
+
+
+class Context:
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().jgpm_init(device, ctypes.byref(self._h)), "init")
+
+    def close(self):
+        if self._h:
+            lib().jgpm_shutdown(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        cu, lds, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        name = ctypes.create_string_buffer(128)
+        _check(lib().jgpm_device_info(self._h, ctypes.byref(cu), ctypes.byref(lds), ctypes.byref(hbm), name, 128),
+               "device_info")
+        return {"num_cu": cu.value, "lds_bytes": lds.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
+
+    def tuning(self):
+        t = _Tuning()
+        _check(lib().jgpm_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
+        return {f: getattr(t, f) for f, _ in _Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = _Tuning(**{f: int(kw.get(f, 0)) for f, _ in _Tuning._fields_})
+        _check(lib().jgpm_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
+
+    def upload(self, csr, flags=SYM):
+        return Graph(self, csr, flags)
+
+    def load(self, prefix, dag_only=False):
+        g = Graph.__new__(Graph)
+        g.ctx = self
+        g._h = ctypes.c_void_p()
+        _check(lib().jgpm_graph_load(self._h, prefix.encode(), int(dag_only), ctypes.byref(g._h)), "graph_load")
+        return g
+
+    def count_batch(self, queries):
+        """queries: list of (Graph, pattern, root_begin, root_end). Returns (counts, kernel_ms)."""
+        arr = (_Query * len(queries))()
+        for i, (g, p, rb, re) in enumerate(queries):
+            arr[i].graph = g._h
+            arr[i].pattern = pattern_id(p)
+            arr[i].root_begin = rb
+            arr[i].root_end = -1 if re is None else re
+        out = (ctypes.c_uint64 * len(queries))()
+        ms = ctypes.c_double()
+        _check(lib().jgpm_count_batch(self._h, arr, len(queries), out, ctypes.byref(ms)), "count_batch")
+        return [int(x) for x in out], ms.value
+
+
+class Graph:
+    """A CSR graph resident in device memory."""
+
+    def __init__(self, ctx, csr, flags=SYM):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        _check(lib().jgpm_graph_upload(ctx._h, _ptr(csr.rowptr), _ptr(csr.col), csr.nv, csr.nnz, flags,
+                                       ctypes.byref(self._h)), "graph_upload")
+
+    def free(self):
+        if self._h:
+            lib().jgpm_graph_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def info(self):
+        nv, sn, dn, sm, dm = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().jgpm_graph_info(self._h, ctypes.byref(nv), ctypes.byref(sn), ctypes.byref(dn),
+                                     ctypes.byref(sm), ctypes.byref(dm)), "graph_info")
+        return {"nv": nv.value, "sym_nnz": sn.value, "dag_nnz": dn.value, "sym_max_deg": sm.value,
+                "dag_max_deg": dm.value}
+
+    def count(self, pattern, roots=None):
+        """Count `pattern` over roots [b, e) (default: all). Returns (count, kernel_ms)."""
+        rb, re = (0, -1) if roots is None else roots
+        out = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        _check(lib().jgpm_count(self._h, pattern_id(pattern), rb, re, ctypes.byref(out), ctypes.byref(ms)),
+               f"count {pattern}")
+        return int(out.value), ms.value
+
+    def work_bytes(self, pattern, roots=None):
+        rb, re = (0, -1) if roots is None else roots
+        out = ctypes.c_uint64()
+        _check(lib().jgpm_work_bytes(self._h, pattern_id(pattern), rb, re, ctypes.byref(out)), "work_bytes")
+        return int(out.value)

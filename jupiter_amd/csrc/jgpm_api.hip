@@ -1,0 +1,426 @@
+// jgpm_api.hip — the C ABI of include/jgpm.h: device context, resident
+// graphs, per-query dispatch to the kernels, status codes.
+//
+// Replaces the reference driver's per-run setup and dispatch
+// (SubgraphSolver, src/jupiter_gpm.cu:27-925): one context per device owns
+// one HIP stream; graphs stay resident in HBM between queries; a query picks
+// kernels by pattern id instead of by string compare, and returns a status
+// instead of asserting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "jgpm_internal.hpp"
+#include "kernels.hpp"
+#include "jgpm_device.hpp"
+
+using namespace jgpm;
+
+#define HIP_TRY(x)                         \
+  do {                                     \
+    hipError_t _e = (x);                   \
+    if (_e != hipSuccess) return JGPM_E_HIP; \
+  } while (0)
+
+extern "C" const char* jgpm_strerror(int s) {
+  switch (s) {
+    case JGPM_OK: return "ok";
+    case JGPM_E_INVALID: return "invalid argument";
+    case JGPM_E_PATTERN: return "unknown pattern id";
+    case JGPM_E_NOMEM: return "out of memory";
+    case JGPM_E_HIP: return "HIP runtime error";
+    case JGPM_E_IO: return "graph file missing or malformed";
+    case JGPM_E_BUDGET: return "request exceeds the memory budget or a supported size";
+    case JGPM_E_GRAPHKIND: return "pattern needs the symmetric graph, which this handle does not hold";
+    case JGPM_E_BUSY: return "queue full";
+    case JGPM_E_NOTREADY: return "result not ready";
+    case JGPM_E_STATE: return "call not valid in the current state";
+    case JGPM_E_NODEVICE: return "no HIP device";
+  }
+  return "unknown status";
+}
+
+// ---------------------------------------------------------------------------
+// small helper kernels
+
+__global__ void k_fill_u64(uint64_t* p, int n, uint64_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+namespace jgpm {
+
+int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(CTX_SLOTS), 0, st, c->d_out, CTX_SLOTS, 0ull);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(CTX_SLOTS), 0, st, c->d_q, CTX_SLOTS, 0u);
+  return hipGetLastError() == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+}
+
+int ensure_scratch(jgpm_ctx_s* c, size_t words) {
+  if (c->scr_words >= words) return JGPM_OK;
+  if (c->d_scr) {
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipFree(c->d_scr));
+    c->d_scr = nullptr;
+    c->scr_words = 0;
+  }
+  if (hipMalloc(&c->d_scr, words * sizeof(uint64_t)) != hipSuccess) return JGPM_E_NOMEM;
+  c->scr_words = words;
+  return JGPM_OK;
+}
+
+static int upload_csr(DevCSR& d, const int64_t* rp, const int32_t* ci, int32_t nv, int64_t nnz,
+                      int32_t max_deg) {
+  d.nv = nv;
+  d.nnz = nnz;
+  d.max_deg = max_deg;
+  if (hipMalloc(&d.rp, sizeof(int64_t) * (nv + 1)) != hipSuccess) return JGPM_E_NOMEM;
+  if (hipMalloc(&d.ci, sizeof(int32_t) * std::max<int64_t>(nnz, 1)) != hipSuccess) return JGPM_E_NOMEM;
+  HIP_TRY(hipMemcpy(d.rp, rp, sizeof(int64_t) * (nv + 1), hipMemcpyHostToDevice));
+  if (nnz) HIP_TRY(hipMemcpy(d.ci, ci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  return JGPM_OK;
+}
+
+static void free_csr(DevCSR& d) {
+  if (d.rp) (void)hipFree(d.rp);
+  if (d.ci) (void)hipFree(d.ci);
+  d = DevCSR{};
+}
+
+// Root tier lists of the DAG: vertex ids (ascending) whose out-degree falls
+// in each block tier; the wave tier scans the root range directly.
+static int build_tiers(jgpm_graph_s* g, const int64_t* rp, int32_t nv) {
+  for (int t = 0; t < jk::NUM_TIERS; ++t) g->tier_h[t].clear();
+  for (int32_t v = 0; v < nv; ++v) {
+    int64_t d = rp[v + 1] - rp[v];
+    if (d <= jk::tier_max_deg(jk::TIER_WAVE)) continue;
+    int t = jk::TIER_LDS2;
+    while (t < jk::NUM_TIERS - 1 && d > jk::tier_max_deg(t)) ++t;
+    if (d > jk::tier_max_deg(t)) {
+      g->huge_h.push_back(v);
+      continue;
+    }
+    g->tier_h[t].push_back(v);
+  }
+  for (int t = 0; t < jk::NUM_TIERS; ++t) {
+    if (g->tier_h[t].empty()) continue;
+    size_t n = g->tier_h[t].size();
+    if (hipMalloc(&g->tier_d[t], n * sizeof(int32_t)) != hipSuccess) return JGPM_E_NOMEM;
+    HIP_TRY(hipMemcpy(g->tier_d[t], g->tier_h[t].data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    g->bytes += (int64_t)(n * sizeof(int32_t));
+  }
+  return JGPM_OK;
+}
+
+static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_s* dag, jgpm_graph_t* out) {
+  auto* g = new (std::nothrow) jgpm_graph_s();
+  if (!g) return JGPM_E_NOMEM;
+  g->ctx = c;
+  int st = JGPM_OK;
+  HIP_TRY(hipSetDevice(c->dev));
+  if (sym) {
+    st = upload_csr(g->sym, sym->rowptr.data(), sym->col.data(), sym->nv, sym->nnz, sym->max_deg);
+    if (st) { jgpm_graph_free(g); return st; }
+    g->has_sym = true;
+    g->bytes += 8 * (int64_t)(sym->nv + 1) + 4 * sym->nnz;
+  }
+  st = upload_csr(g->dag, dag->rowptr.data(), dag->col.data(), dag->nv, dag->nnz, dag->max_deg);
+  if (st) { jgpm_graph_free(g); return st; }
+  g->has_dag = true;
+  g->bytes += 8 * (int64_t)(dag->nv + 1) + 4 * dag->nnz;
+  st = build_tiers(g, dag->rowptr.data(), dag->nv);
+  if (st) { jgpm_graph_free(g); return st; }
+  *out = g;
+  return JGPM_OK;
+}
+
+}  // namespace jgpm
+
+// ---------------------------------------------------------------------------
+extern "C" int jgpm_init(int dev, jgpm_ctx_t* out) {
+  if (!out) return JGPM_E_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return JGPM_E_NODEVICE;
+  if (dev < 0 || dev >= n) return JGPM_E_INVALID;
+  HIP_TRY(hipSetDevice(dev));
+  auto* c = new (std::nothrow) jgpm_ctx_s();
+  if (!c) return JGPM_E_NOMEM;
+  c->dev = dev;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) { delete c; return JGPM_E_HIP; }
+  c->num_cu = p.multiProcessorCount;
+  c->lds = (int)p.sharedMemPerBlock;
+  c->hbm = (int64_t)p.totalGlobalMem;
+  snprintf(c->name, sizeof(c->name), "%s", p.gcnArchName);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return JGPM_E_HIP; }
+  if (hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+    delete c;
+    return JGPM_E_HIP;
+  }
+  if (hipMalloc(&c->d_out, CTX_SLOTS * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&c->d_q, CTX_SLOTS * sizeof(uint32_t)) != hipSuccess) {
+    delete c;
+    return JGPM_E_NOMEM;
+  }
+  c->tune.small_max_deg = 64;
+  c->tune.lds_max_deg = 512;
+  c->tune.gallop_ratio = 32;
+  c->tune.waves_per_block = 4;
+  c->tune.blocks_per_cu = 8;
+  *out = c;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_shutdown(jgpm_ctx_t c) {
+  if (!c) return JGPM_E_INVALID;
+  (void)hipSetDevice(c->dev);
+  (void)hipStreamSynchronize(c->st);
+  if (c->d_scr) (void)hipFree(c->d_scr);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->d_q) (void)hipFree(c->d_q);
+  (void)hipEventDestroy(c->e0);
+  (void)hipEventDestroy(c->e1);
+  (void)hipStreamDestroy(c->st);
+  delete c;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_device_info(jgpm_ctx_t c, int* num_cu, int* lds_bytes, int64_t* hbm_bytes, char* name,
+                                int name_len) {
+  if (!c) return JGPM_E_INVALID;
+  if (num_cu) *num_cu = c->num_cu;
+  if (lds_bytes) *lds_bytes = c->lds;
+  if (hbm_bytes) *hbm_bytes = c->hbm;
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_get_tuning(jgpm_ctx_t c, jgpm_tuning* t) {
+  if (!c || !t) return JGPM_E_INVALID;
+  *t = c->tune;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_set_tuning(jgpm_ctx_t c, const jgpm_tuning* t) {
+  if (!c || !t) return JGPM_E_INVALID;
+  if (t->gallop_ratio > 0) c->tune.gallop_ratio = t->gallop_ratio;
+  if (t->blocks_per_cu > 0) c->tune.blocks_per_cu = std::min(t->blocks_per_cu, 64);
+  if (t->small_max_deg > 0) c->tune.small_max_deg = std::min(t->small_max_deg, 64);
+  if (t->lds_max_deg > 0) c->tune.lds_max_deg = t->lds_max_deg;
+  if (t->waves_per_block > 0) c->tune.waves_per_block = t->waves_per_block;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_graph_upload(jgpm_ctx_t c, const int64_t* rowptr, const int32_t* colidx, int32_t nv,
+                                 int64_t nnz, uint32_t flags, jgpm_graph_t* out) {
+  if (!c || !rowptr || !out || nv < 0 || nnz < 0 || (nnz > 0 && !colidx)) return JGPM_E_INVALID;
+  if (flags != JGPM_SYM && flags != JGPM_DAG) return JGPM_E_INVALID;
+  if (rowptr[0] != 0 || rowptr[nv] != nnz) return JGPM_E_INVALID;
+  jgpm_csr_s h;
+  h.nv = nv;
+  h.nnz = nnz;
+  try {
+    h.rowptr.assign(rowptr, rowptr + nv + 1);
+    h.col.assign(colidx, colidx + nnz);
+  } catch (const std::bad_alloc&) {
+    return JGPM_E_NOMEM;
+  }
+  for (int32_t v = 0; v < nv; ++v) {
+    if (h.rowptr[v + 1] < h.rowptr[v]) return JGPM_E_INVALID;
+    for (int64_t k = h.rowptr[v]; k < h.rowptr[v + 1]; ++k) {
+      if (h.col[k] < 0 || h.col[k] >= nv) return JGPM_E_INVALID;
+      if (k > h.rowptr[v] && h.col[k] <= h.col[k - 1]) return JGPM_E_INVALID;  // ascending, no duplicates
+    }
+  }
+  h.max_deg = compute_max_degree(h);
+  if (flags == JGPM_DAG) return graph_from_host(c, nullptr, &h, out);
+  jgpm_csr_s dag;
+  try {
+    orient_dag(h, dag);
+  } catch (const std::bad_alloc&) {
+    return JGPM_E_NOMEM;
+  }
+  return graph_from_host(c, &h, &dag, out);
+}
+
+extern "C" int jgpm_graph_load(jgpm_ctx_t c, const char* prefix, int dag_only, jgpm_graph_t* out) {
+  if (!c || !prefix || !out) return JGPM_E_INVALID;
+  jgpm_csr_s h;
+  int st = read_csr(prefix, h);
+  if (st) return st;
+  jgpm_csr_s dag;
+  try {
+    orient_dag(h, dag);
+  } catch (const std::bad_alloc&) {
+    return JGPM_E_NOMEM;
+  }
+  return graph_from_host(c, dag_only ? nullptr : &h, &dag, out);
+}
+
+extern "C" int jgpm_graph_free(jgpm_graph_t g) {
+  if (!g) return JGPM_E_INVALID;
+  (void)hipSetDevice(g->ctx->dev);
+  (void)hipStreamSynchronize(g->ctx->st);
+  free_csr(g->sym);
+  free_csr(g->dag);
+  for (int t = 0; t < jk::NUM_TIERS; ++t)
+    if (g->tier_d[t]) (void)hipFree(g->tier_d[t]);
+  delete g;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_graph_info(jgpm_graph_t g, int32_t* nv, int64_t* sym_nnz, int64_t* dag_nnz,
+                               int32_t* sym_max_deg, int32_t* dag_max_deg) {
+  if (!g) return JGPM_E_INVALID;
+  if (nv) *nv = g->dag.nv;
+  if (sym_nnz) *sym_nnz = g->has_sym ? g->sym.nnz : -1;
+  if (dag_nnz) *dag_nnz = g->dag.nnz;
+  if (sym_max_deg) *sym_max_deg = g->has_sym ? g->sym.max_deg : -1;
+  if (dag_max_deg) *dag_max_deg = g->dag.max_deg;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_graph_bytes(jgpm_graph_t g, int64_t* bytes) {
+  if (!g || !bytes) return JGPM_E_INVALID;
+  *bytes = g->bytes;
+  return JGPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Counting
+
+namespace jgpm {
+
+int clique_k(int pattern) {
+  switch (pattern) {
+    case JGPM_TC: return 3;
+    case JGPM_P13: case JGPM_I_K4: return 4;
+    case JGPM_P14: return 5;
+    case JGPM_P15: return 6;
+    case JGPM_P16: return 7;
+  }
+  return 0;
+}
+
+// Enqueue the clique engine for roots [rb, re) accumulating into out_slot.
+int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st) {
+  jgpm_ctx_s* c = g->ctx;
+  if (!g->huge_h.empty()) {
+    auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
+    if (lo != g->huge_h.end() && *lo < re) return JGPM_E_BUDGET;
+  }
+  const int maxgrid = std::max(1, c->tune.blocks_per_cu) * c->num_cu;
+  for (int tier = jk::NUM_TIERS - 1; tier >= 0; --tier) {
+    jk::CliqueArgs a{};
+    a.rp = g->dag.rp;
+    a.ci = g->dag.ci;
+    a.ratio = std::max(1, c->tune.gallop_ratio);
+    a.qctr = c->d_q + (out_slot * jk::NUM_TIERS + tier) % CTX_SLOTS;
+    a.out = c->d_out + out_slot;
+    a.gscr = nullptr;
+    a.dmax = jk::tier_max_deg(tier);
+    a.dmin = tier == jk::TIER_WAVE ? (bytes ? 0 : k - 1) : jk::tier_max_deg(tier - 1) + 1;
+    int grid;
+    if (tier == jk::TIER_WAVE) {
+      a.ids = nullptr;
+      a.rb = rb;
+      a.ntasks = re - rb;
+      a.chunk = 16;
+      if (a.ntasks <= 0) continue;
+      int64_t waves = (a.ntasks + a.chunk - 1) / a.chunk;
+      grid = (int)std::min<int64_t>(maxgrid, (waves + 3) / 4);
+    } else {
+      const auto& v = g->tier_h[tier];
+      auto lo = std::lower_bound(v.begin(), v.end(), (int32_t)rb);
+      auto hi = std::lower_bound(v.begin(), v.end(), (int32_t)std::min<int64_t>(re, INT32_MAX));
+      int64_t n = hi - lo;
+      if (n <= 0) continue;
+      a.ids = g->tier_d[tier] + (lo - v.begin());
+      a.rb = 0;
+      a.ntasks = n;
+      a.chunk = 1;
+      grid = (int)std::min<int64_t>(maxgrid, n);
+      size_t sw = jk::tier_scratch_words(tier);
+      if (sw) {
+        grid = (int)std::min<int64_t>(grid, 4 * (int64_t)c->num_cu);
+        int s2 = ensure_scratch(c, sw * (size_t)grid);
+        if (s2) return s2;
+        a.gscr = c->d_scr;
+      }
+    }
+    if (jk::launch_clique(k, bytes, tier, a, grid, st) != hipSuccess) return JGPM_E_HIP;
+  }
+  return JGPM_OK;
+}
+
+}  // namespace jgpm
+
+static int check_range(jgpm_graph_s* g, int64_t& rb, int64_t& re) {
+  const int32_t nv = g->dag.nv;
+  if (re < 0) re = nv;
+  if (rb < 0 || rb > re || re > nv) return JGPM_E_INVALID;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end, uint64_t* count,
+                          double* kernel_ms) {
+  if (!g || !count) return JGPM_E_INVALID;
+  if (pattern < 0 || pattern >= JGPM_NUM_PATTERNS) return JGPM_E_PATTERN;
+  int st = check_range(g, root_begin, root_end);
+  if (st) return st;
+  jgpm_ctx_s* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipEventRecord(c->e0, c->st));
+  st = ctx_reset_counters(c, c->st);
+  if (st) return st;
+  st = enqueue_pattern(g, pattern, root_begin, root_end, 0, c->st);
+  if (st) {
+    (void)hipStreamSynchronize(c->st);
+    return st;
+  }
+  HIP_TRY(hipEventRecord(c->e1, c->st));
+  uint64_t h[CTX_SLOTS];
+  HIP_TRY(hipMemcpyAsync(h, c->d_out, sizeof(h), hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  *count = finish_pattern(pattern, h);
+  if (kernel_ms) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+    *kernel_ms = ms;
+  }
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_work_bytes(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end, uint64_t* bytes) {
+  if (!g || !bytes) return JGPM_E_INVALID;
+  int k = clique_k(pattern);
+  if (!k || pattern == JGPM_I_K4) return JGPM_E_PATTERN;
+  int st = check_range(g, root_begin, root_end);
+  if (st) return st;
+  jgpm_ctx_s* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->dev));
+  st = ctx_reset_counters(c, c->st);
+  if (st) return st;
+  st = enqueue_clique(g, k, true, root_begin, root_end, 0, c->st);
+  if (st) {
+    (void)hipStreamSynchronize(c->st);
+    return st;
+  }
+  uint64_t h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, c->d_out, sizeof(h), hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  *bytes = h;
+  return JGPM_OK;
+}

@@ -1,0 +1,47 @@
+// Internal declarations shared by the host-side translation units of libjgpm.
+#pragma once
+#include <atomic>
+#include <cstdint>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#include "jgpm.h"
+
+// Host CSR: int64 row pointers and int32 ascending column indices, the
+// on-disk and in-memory layout of the reference's Graph (include/graph.h:21-23).
+struct jgpm_csr_s {
+  int32_t nv = 0;
+  int64_t nnz = 0;
+  int32_t max_deg = 0;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> col;
+};
+
+namespace jgpm {
+
+int host_threads();
+
+// Dynamic-chunk parallel loop over [0, n) on host threads; f(begin, end).
+void parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& f);
+
+// splitmix64: the integer PRNG behind every generator (portable, seedable,
+// counter-based so results do not depend on the thread count).
+static inline uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static inline uint64_t mix_seed(uint64_t seed, uint64_t stream) {
+  uint64_t s = seed ^ (stream * 0xD1B54A32D192ED03ull);
+  return splitmix64(s);
+}
+
+int build_sym_csr(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst, jgpm_csr_s& out);
+void orient_dag(const jgpm_csr_s& sym, jgpm_csr_s& dag);
+int32_t compute_max_degree(const jgpm_csr_s& c);
+int write_csr(const jgpm_csr_s& c, const char* prefix);
+int read_csr(const char* prefix, jgpm_csr_s& c);
+
+}  // namespace jgpm
