@@ -1,0 +1,137 @@
+"""Benchmark: 4-clique (P13) and 5-clique (P14) counting over one quarter of
+the roots of a seeded R-MAT graph (scale 22, edge factor 16, a/b/c/d =
+0.57/0.19/0.19/0.05, degree-ordered DAG, int32 CSR, uint64 counts).
+
+A step is one P13 query plus one P14 query over roots [0, 2^20), each a
+plain per-call launch that returns its exact count to the host.  `value` is
+queries/s over the timed steps (inputs resident in HBM before timing).
+
+Also reported on the same JSON line:
+  roofline      the dominant kernel (largest device time over the timed
+                region, from HIP events on the library's stream): work-model
+                bytes of its launch (every neighbour-list fetch of the
+                reference's loop nest costs 16 + 4 d(v) bytes) / its average
+                launch time, against the 8 TB/s HBM peak;
+  cpu_baseline  the CPU oracle (clean-room OpenMP restatement of the
+                reference nests) timed on a seeded random sample of the same
+                roots, scaled to the full query;
+  latency       p50 / p99 per query kind over the timed steps.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import jupiter_amd as J  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "queries/s served vs plain per-call kernel (bit-exact counts); p50/p99; GB/s"
+SCALE, EF, SEED = 22, 16, 1
+QUERIES = ("P13", "P14")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(dag, roots_end, frac_den=1024, seed=7):
+    """Oracle on a seeded uniform sample of 1/frac_den of the roots, both
+    query kinds; scaled by frac_den to the full query."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    rng = np.random.default_rng(seed)
+    roots = np.sort(rng.choice(roots_end, size=roots_end // frac_den, replace=False)).astype(np.int32)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t_total = 0.0
+    for p in QUERIES:
+        t0 = time.perf_counter()
+        O.count_roots(dag.rowptr, dag.col, J.PATTERNS[p], roots, mode=1, threads=threads)
+        t_total += time.perf_counter() - t0
+    est = t_total * frac_den  # seconds for one P13 + one P14 over all roots
+    return {"value": round(len(QUERIES) / est, 6), "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{len(roots)} roots (uniform seeded 1/{frac_den} of [0,{roots_end})), P13+P14, "
+                      f"{t_total:.1f}s measured, scaled x{frac_den}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scale", type=int, default=SCALE)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the count shards by root "
+              "range; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    g = J.rmat(args.scale, EF, 0.57, 0.19, 0.19, seed=SEED)
+    log(f"R-MAT scale {args.scale}: nv={g.nv} sym_nnz={g.nnz} ({time.time() - t0:.1f}s)")
+    q = g.nv // 4
+    with J.Context(0) as ctx:
+        G = ctx.upload(g)
+        info = G.info()
+        log(f"device {ctx.info()} graph {info}")
+        for w in range(args.warmup):
+            for p in QUERIES:
+                c, ms = G.count(p, (0, q))
+                log(f"warmup {w} {p} count={c} {ms:.2f} ms")
+        counts = {}
+        lat = {p: [] for p in QUERIES}
+        kern = {}  # (pattern, kernel index, name) -> list of ms
+        t_start = time.perf_counter()
+        for s in range(args.steps):
+            for p in QUERIES:
+                ts = time.perf_counter()
+                c, ms = G.count(p, (0, q))
+                lat[p].append((time.perf_counter() - ts) * 1e3)
+                if counts.setdefault(p, c) != c:
+                    raise AssertionError(f"{p} count changed between steps: {counts[p]} != {c}")
+                for i, (name, kms) in enumerate(ctx.last_profile()):
+                    kern.setdefault((p, i, name), []).append(kms)
+            log(f"step {s} done")
+        elapsed = time.perf_counter() - t_start
+        n_queries = args.steps * len(QUERIES)
+        value = n_queries / elapsed
+        # dominant kernel and its work-model bytes
+        (dp, di, dname), dms = max(kern.items(), key=lambda kv: sum(kv[1]))
+        split = G.work_bytes_split(dp, (0, q))
+        dbytes = split[di]
+        avg_ms = float(np.mean(dms))
+        achieved = dbytes / (avg_ms * 1e-3) / 1e9
+        total_bytes = {p: G.work_bytes(p, (0, q)) for p in QUERIES}
+        G.free()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"{dname} ({dp})", "kernel_avg_ms": round(avg_ms, 3), "algorithmic_bytes": dbytes}
+    line = {
+        "metric": METRIC, "value": round(value, 4), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"P13+P14 count, roots [0,2^{args.scale - 2}) of R-MAT scale {args.scale} "
+                               f"ef {EF} (0.57/0.19/0.19/0.05, seed {SEED}), degree DAG",
+                   "nv": info["nv"], "sym_nnz": info["sym_nnz"], "dag_nnz": info["dag_nnz"],
+                   "dag_max_deg": info["dag_max_deg"], "counts": counts,
+                   "latency_ms": {p: {"p50": round(float(np.percentile(v, 50)), 3),
+                                      "p99": round(float(np.percentile(v, 99)), 3)} for p, v in lat.items()},
+                   "kernels_ms": {f"{p}:{n}": round(float(np.mean(v)), 3) for (p, i, n), v in kern.items()},
+                   "work_bytes": total_bytes},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(g.orient(), q)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -118,6 +118,15 @@ int  jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_en
 int  jgpm_work_bytes(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
                      uint64_t* bytes);
 
+/* Per-kernel profile of the last jgpm_count on this context: the device
+ * time (HIP events on the context stream) of every kernel launch, in launch
+ * order, with a static name for each.  Returns the number of launches. */
+int  jgpm_last_profile(jgpm_ctx_t ctx, int n, const char** names, double* ms);
+/* jgpm_work_bytes split over the same kernel launches as jgpm_count uses
+ * for the query (order matches jgpm_last_profile).  Returns the number. */
+int  jgpm_work_bytes_split(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
+                           uint64_t* bytes, int n);
+
 /* Tuning knobs (autotuner output); 0 keeps the current value. */
 typedef struct {
   int32_t small_max_deg;    /* roots with out-degree <= this use one wave each (<= 64)   */

@@ -96,6 +96,9 @@ def lib():
                        ctypes.c_int),
         "jgpm_work_bytes": ([P, ctypes.c_int, I64, I64, ctypes.POINTER(U64)], ctypes.c_int),
         "jgpm_get_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "jgpm_last_profile": ([P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double)],
+                              ctypes.c_int),
+        "jgpm_work_bytes_split": ([P, ctypes.c_int, I64, I64, ctypes.POINTER(U64), ctypes.c_int], ctypes.c_int),
         "jgpm_set_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
         "jgpm_count_batch": ([P, ctypes.POINTER(_Query), ctypes.c_int, ctypes.POINTER(U64),
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -277,6 +280,15 @@ class Context:
         t = _Tuning(**{f: int(kw.get(f, 0)) for f, _ in _Tuning._fields_})
         _check(lib().jgpm_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
+    def last_profile(self):
+        """[(kernel name, ms)] of the last count on this context, in launch order."""
+        names = (ctypes.c_char_p * 16)()
+        ms = (ctypes.c_double * 16)()
+        n = lib().jgpm_last_profile(self._h, 16, names, ms)
+        if n < 0:
+            raise JgpmError(n, "last_profile")
+        return [(names[i].decode(), ms[i]) for i in range(min(n, 16))]
+
     def upload(self, csr, flags=SYM):
         return Graph(self, csr, flags)
 
@@ -330,6 +342,15 @@ class Graph:
         _check(lib().jgpm_count(self._h, pattern_id(pattern), rb, re, ctypes.byref(out), ctypes.byref(ms)),
                f"count {pattern}")
         return int(out.value), ms.value
+
+    def work_bytes_split(self, pattern, roots=None):
+        """Work-model bytes per kernel launch of the query (order of last_profile)."""
+        rb, re = (0, -1) if roots is None else roots
+        out = (ctypes.c_uint64 * 16)()
+        n = lib().jgpm_work_bytes_split(self._h, pattern_id(pattern), rb, re, out, 16)
+        if n < 0:
+            raise JgpmError(n, "work_bytes_split")
+        return [int(out[i]) for i in range(min(n, 16))]
 
     def work_bytes(self, pattern, roots=None):
         rb, re = (0, -1) if roots is None else roots

@@ -6,14 +6,106 @@
 // routine and an unknown id is JGPM_E_PATTERN.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "jgpm_device.hpp"
 
 namespace jgpm {
 
+static int grid_for(jgpm_ctx_s* c, int per_cu) { return std::max(1, per_cu) * c->num_cu; }
+
+// Lazily built per-graph arrays of the symmetric kernels.
+static int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st) {
+  jgpm_ctx_s* c = g->ctx;
+  const int64_t ne = g->sym.nnz;
+  if (!g->d_src) {
+    if (hipMalloc(&g->d_src, sizeof(int32_t) * std::max<int64_t>(ne, 1)) != hipSuccess) return JGPM_E_NOMEM;
+    g->bytes += 4 * ne;
+    if (jk::launch_aux_src(g->sym.rp, g->sym.nv, g->d_src, grid_for(c, 4), st) != hipSuccess) return JGPM_E_HIP;
+  }
+  if (need_tri && !g->d_tri) {
+    if (hipMalloc(&g->d_tri, sizeof(int32_t) * std::max<int64_t>(ne, 1)) != hipSuccess) return JGPM_E_NOMEM;
+    g->bytes += 4 * ne;
+    unsigned long long* q = c->d_q64 + (CTX_SLOTS - 1);
+    prof_begin(c, st, "edge_triangles");
+    hipError_t e = jk::launch_aux_tri(g->sym.rp, g->sym.ci, g->d_src, ne, g->d_tri, q, grid_for(c, 4), st);
+    prof_end(c, st);
+    if (e != hipSuccess) return JGPM_E_HIP;
+  }
+  if (need_nsum && !g->d_nsum) {
+    if (hipMalloc(&g->d_nsum, sizeof(int64_t) * (g->sym.nv + 1)) != hipSuccess) return JGPM_E_NOMEM;
+    g->bytes += 8 * (int64_t)g->sym.nv;
+    if (jk::launch_aux_nsum(g->sym.rp, g->sym.ci, g->sym.nv, g->d_nsum, grid_for(c, 4), st) != hipSuccess)
+      return JGPM_E_HIP;
+  }
+  return JGPM_OK;
+}
+
+static bool uses_tri(int p) { return p == 3 || p == 4 || p == 7 || p == 10 || p == 11; }
+
+int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStream_t st) {
+  if (!g->has_sym) return JGPM_E_GRAPHKIND;
+  jgpm_ctx_s* c = g->ctx;
+  // The triangle array is shared by later queries; its counter slot is
+  // reset per query by ctx_reset_counters before it is used here.
+  int rc = ensure_sym_aux(g, uses_tri(p), p == 3, st);
+  if (rc) return rc;
+  const int wpb = jk::sym_waves_per_block();
+  const int grid = grid_for(c, 2);
+  const size_t stride = (size_t)std::max(1, g->sym.max_deg);
+  const size_t need = (size_t)grid * wpb * jk::sym_lists_per_wave() * stride;
+  if (c->sscr_ints < need) {
+    if (c->d_sscr) {
+      if (hipStreamSynchronize(st) != hipSuccess) return JGPM_E_HIP;
+      (void)hipFree(c->d_sscr);
+      c->d_sscr = nullptr;
+      c->sscr_ints = 0;
+    }
+    if (hipMalloc(&c->d_sscr, need * sizeof(int32_t)) != hipSuccess) return JGPM_E_NOMEM;
+    c->sscr_ints = need;
+  }
+  jk::SymArgs a{};
+  a.rp = g->sym.rp;
+  a.ci = g->sym.ci;
+  a.src = g->d_src;
+  a.tri = g->d_tri;
+  a.nsum = g->d_nsum;
+  a.e0 = g->sym_rp_h[rb];
+  a.ne = g->sym_rp_h[re] - a.e0;
+  if (a.ne <= 0) return JGPM_OK;
+  a.qctr = c->d_q64 + (slot % (CTX_SLOTS - 1));
+  a.chunk = (p == 2 || p == 3) ? 64 : 8;
+  a.scr = c->d_sscr;
+  a.scr_stride = (int64_t)stride;
+  a.out = c->d_out + slot;
+  static const char* const kSymName[13] = {"", "sym_p1", "sym_p2", "sym_p3", "sym_p4", "sym_p5", "sym_p6",
+                                           "sym_p7", "sym_p8", "sym_p9", "sym_p10", "sym_p11", "sym_p12"};
+  prof_begin(c, st, kSymName[p]);
+  hipError_t e = jk::launch_sym(p, a, grid, st);
+  prof_end(c, st);
+  return e == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+}
+
+int enqueue_vertex_sum(jgpm_graph_s* g, int mode, int64_t rb, int64_t re, int slot, hipStream_t st) {
+  if (!g->has_sym) return JGPM_E_GRAPHKIND;
+  int rc = ensure_sym_aux(g, mode == 1, false, st);
+  if (rc) return rc;
+  if (re <= rb) return JGPM_OK;
+  return jk::launch_vertex_sums(g->sym.rp, g->d_tri, rb, re, mode, (unsigned long long*)(g->ctx->d_out + slot),
+                                grid_for(g->ctx, 4), st) == hipSuccess
+             ? JGPM_OK
+             : JGPM_E_HIP;
+}
+
+// Induced 4-vertex motifs from non-induced counts (exact inverse of the
+// conversion matrix): with S = 3-star, T = tailed triangle, C = 4-cycle (P1),
+// D = diamond (P2), K = K4 (P13):
+//   I_K = K, I_D = D - 6K, I_C = C - D + 3K, I_T = T - 4D + 12K,
+//   I_S = S - T + 2D - 4K.
 int pattern_slots(int pattern) {
   switch (pattern) {
-    case JGPM_I_C4: case JGPM_I_DIAMOND: return 3;
-    case JGPM_I_TAILTRI: return 3;
+    case JGPM_I_DIAMOND: return 2;
+    case JGPM_I_C4: case JGPM_I_TAILTRI: return 3;
     case JGPM_I_STAR3: return 4;
   }
   return 1;
@@ -25,11 +117,40 @@ int enqueue_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re, int s,
     if (!g->has_dag) return JGPM_E_GRAPHKIND;
     return enqueue_clique(g, k, false, rb, re, s, st);
   }
+  if (pattern >= JGPM_P1 && pattern <= JGPM_P12) return enqueue_sym(g, pattern, rb, re, s, st);
+  if (!g->has_sym) return JGPM_E_GRAPHKIND;
+  int rc = JGPM_OK;
+  switch (pattern) {
+    case JGPM_STAR3: return enqueue_vertex_sum(g, 0, rb, re, s, st);
+    case JGPM_TAILTRI: return enqueue_vertex_sum(g, 1, rb, re, s, st);
+    case JGPM_I_DIAMOND:
+      if ((rc = enqueue_sym(g, JGPM_P2, rb, re, s, st))) return rc;
+      return enqueue_clique(g, 4, false, rb, re, s + 1, st);
+    case JGPM_I_C4:
+      if ((rc = enqueue_sym(g, JGPM_P1, rb, re, s, st))) return rc;
+      if ((rc = enqueue_sym(g, JGPM_P2, rb, re, s + 1, st))) return rc;
+      return enqueue_clique(g, 4, false, rb, re, s + 2, st);
+    case JGPM_I_TAILTRI:
+      if ((rc = enqueue_vertex_sum(g, 1, rb, re, s, st))) return rc;
+      if ((rc = enqueue_sym(g, JGPM_P2, rb, re, s + 1, st))) return rc;
+      return enqueue_clique(g, 4, false, rb, re, s + 2, st);
+    case JGPM_I_STAR3:
+      if ((rc = enqueue_vertex_sum(g, 0, rb, re, s, st))) return rc;
+      if ((rc = enqueue_vertex_sum(g, 1, rb, re, s + 1, st))) return rc;
+      if ((rc = enqueue_sym(g, JGPM_P2, rb, re, s + 2, st))) return rc;
+      return enqueue_clique(g, 4, false, rb, re, s + 3, st);
+  }
   return JGPM_E_PATTERN;
 }
 
 uint64_t finish_pattern(int pattern, const uint64_t* h) {
-  (void)pattern;
+  // unsigned wrap-around arithmetic is exact for the non-negative result
+  switch (pattern) {
+    case JGPM_I_DIAMOND: return h[0] - 6 * h[1];
+    case JGPM_I_C4: return h[0] - h[1] + 3 * h[2];
+    case JGPM_I_TAILTRI: return h[0] - 4 * h[1] + 12 * h[2];
+    case JGPM_I_STAR3: return h[0] - h[1] + 2 * h[2] - 4 * h[3];
+  }
   return h[0];
 }
 

@@ -63,7 +63,19 @@ namespace jgpm {
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(CTX_SLOTS), 0, st, c->d_out, CTX_SLOTS, 0ull);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(CTX_SLOTS), 0, st, c->d_q, CTX_SLOTS, 0u);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(CTX_SLOTS), 0, st, (uint64_t*)c->d_q64, CTX_SLOTS, 0ull);
   return hipGetLastError() == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+}
+
+void prof_begin(jgpm_ctx_s* c, hipStream_t st, const char* name) {
+  if (!c->prof_on || st != c->st || c->nprof >= jgpm_ctx_s::MAX_PROF) return;
+  c->pname[c->nprof] = name;
+  (void)hipEventRecord(c->pev[2 * c->nprof], st);
+}
+void prof_end(jgpm_ctx_s* c, hipStream_t st) {
+  if (!c->prof_on || st != c->st || c->nprof >= jgpm_ctx_s::MAX_PROF) return;
+  (void)hipEventRecord(c->pev[2 * c->nprof + 1], st);
+  c->nprof++;
 }
 
 int ensure_scratch(jgpm_ctx_s* c, size_t words) {
@@ -133,6 +145,7 @@ static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_
     if (st) { jgpm_graph_free(g); return st; }
     g->has_sym = true;
     g->bytes += 8 * (int64_t)(sym->nv + 1) + 4 * sym->nnz;
+    g->sym_rp_h = sym->rowptr;
   }
   st = upload_csr(g->dag, dag->rowptr.data(), dag->col.data(), dag->nv, dag->nnz, dag->max_deg);
   if (st) { jgpm_graph_free(g); return st; }
@@ -167,8 +180,14 @@ extern "C" int jgpm_init(int dev, jgpm_ctx_t* out) {
     delete c;
     return JGPM_E_HIP;
   }
+  for (int i = 0; i < 2 * jgpm_ctx_s::MAX_PROF; ++i)
+    if (hipEventCreate(&c->pev[i]) != hipSuccess) {
+      delete c;
+      return JGPM_E_HIP;
+    }
   if (hipMalloc(&c->d_out, CTX_SLOTS * sizeof(uint64_t)) != hipSuccess ||
-      hipMalloc(&c->d_q, CTX_SLOTS * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&c->d_q, CTX_SLOTS * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->d_q64, CTX_SLOTS * sizeof(unsigned long long)) != hipSuccess) {
     delete c;
     return JGPM_E_NOMEM;
   }
@@ -188,8 +207,11 @@ extern "C" int jgpm_shutdown(jgpm_ctx_t c) {
   if (c->d_scr) (void)hipFree(c->d_scr);
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_q) (void)hipFree(c->d_q);
+  if (c->d_q64) (void)hipFree(c->d_q64);
+  if (c->d_sscr) (void)hipFree(c->d_sscr);
   (void)hipEventDestroy(c->e0);
   (void)hipEventDestroy(c->e1);
+  for (int i = 0; i < 2 * jgpm_ctx_s::MAX_PROF; ++i) (void)hipEventDestroy(c->pev[i]);
   (void)hipStreamDestroy(c->st);
   delete c;
   return JGPM_OK;
@@ -275,6 +297,9 @@ extern "C" int jgpm_graph_free(jgpm_graph_t g) {
   free_csr(g->dag);
   for (int t = 0; t < jk::NUM_TIERS; ++t)
     if (g->tier_d[t]) (void)hipFree(g->tier_d[t]);
+  if (g->d_src) (void)hipFree(g->d_src);
+  if (g->d_tri) (void)hipFree(g->d_tri);
+  if (g->d_nsum) (void)hipFree(g->d_nsum);
   delete g;
   return JGPM_OK;
 }
@@ -313,7 +338,11 @@ int clique_k(int pattern) {
 }
 
 // Enqueue the clique engine for roots [rb, re) accumulating into out_slot.
-int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st) {
+static const char* const kTierName[jk::NUM_TIERS] = {"clique_wave64", "clique_lds128", "clique_lds256",
+                                                     "clique_lds512", "clique_glb2048"};
+
+int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
+                   bool split_slots) {
   jgpm_ctx_s* c = g->ctx;
   if (!g->huge_h.empty()) {
     auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
@@ -326,7 +355,7 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
     a.ci = g->dag.ci;
     a.ratio = std::max(1, c->tune.gallop_ratio);
     a.qctr = c->d_q + (out_slot * jk::NUM_TIERS + tier) % CTX_SLOTS;
-    a.out = c->d_out + out_slot;
+    a.out = c->d_out + out_slot + (split_slots ? tier : 0);
     a.gscr = nullptr;
     a.dmax = jk::tier_max_deg(tier);
     a.dmin = tier == jk::TIER_WAVE ? (bytes ? 0 : k - 1) : jk::tier_max_deg(tier - 1) + 1;
@@ -358,7 +387,10 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
         a.gscr = c->d_scr;
       }
     }
-    if (jk::launch_clique(k, bytes, tier, a, grid, st) != hipSuccess) return JGPM_E_HIP;
+    prof_begin(c, st, kTierName[tier]);
+    hipError_t e = jk::launch_clique(k, bytes, tier, a, grid, st);
+    prof_end(c, st);
+    if (e != hipSuccess) return JGPM_E_HIP;
   }
   return JGPM_OK;
 }
@@ -384,6 +416,7 @@ extern "C" int jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64
   HIP_TRY(hipEventRecord(c->e0, c->st));
   st = ctx_reset_counters(c, c->st);
   if (st) return st;
+  c->nprof = 0;
   st = enqueue_pattern(g, pattern, root_begin, root_end, 0, c->st);
   if (st) {
     (void)hipStreamSynchronize(c->st);
@@ -423,4 +456,49 @@ extern "C" int jgpm_work_bytes(jgpm_graph_t g, int pattern, int64_t root_begin, 
   HIP_TRY(hipStreamSynchronize(c->st));
   *bytes = h;
   return JGPM_OK;
+}
+
+extern "C" int jgpm_last_profile(jgpm_ctx_t c, int n, const char** names, double* ms) {
+  if (!c || n < 0) return JGPM_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  for (int i = 0; i < c->nprof && i < n; ++i) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, c->pev[2 * i], c->pev[2 * i + 1]) != hipSuccess) return JGPM_E_HIP;
+    if (names) names[i] = c->pname[i];
+    if (ms) ms[i] = t;
+  }
+  return c->nprof;
+}
+
+extern "C" int jgpm_work_bytes_split(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
+                                     uint64_t* bytes, int n) {
+  if (!g || !bytes || n < 0) return JGPM_E_INVALID;
+  int k = clique_k(pattern);
+  if (!k || pattern == JGPM_I_K4) return JGPM_E_PATTERN;
+  int st = check_range(g, root_begin, root_end);
+  if (st) return st;
+  jgpm_ctx_s* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->dev));
+  st = ctx_reset_counters(c, c->st);
+  if (st) return st;
+  c->nprof = 0;
+  st = enqueue_clique(g, k, true, root_begin, root_end, 0, c->st, true);
+  if (st) {
+    (void)hipStreamSynchronize(c->st);
+    return st;
+  }
+  uint64_t h[CTX_SLOTS];
+  HIP_TRY(hipMemcpyAsync(h, c->d_out, sizeof(h), hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  // launches run from the last tier down, skipping empty tiers: report in launch order
+  int m = 0;
+  for (int tier = jk::NUM_TIERS - 1; tier >= 0 && m < c->nprof; --tier) {
+    bool launched = false;
+    for (int i = 0; i < c->nprof; ++i) launched |= (c->pname[i] == kTierName[tier]);
+    if (!launched) continue;
+    if (m < n) bytes[m] = h[tier];
+    ++m;
+  }
+  return m;
 }

@@ -30,9 +30,18 @@ struct jgpm_ctx_s {
   jgpm_tuning tune{};
   uint64_t* d_out = nullptr;  // CTX_SLOTS accumulators
   uint32_t* d_q = nullptr;    // CTX_SLOTS work-queue counters
+  unsigned long long* d_q64 = nullptr;  // CTX_SLOTS 64-bit work-queue counters
+  int32_t* d_sscr = nullptr;  // list scratch of the symmetric-pattern kernels
+  size_t sscr_ints = 0;
   uint64_t* d_scr = nullptr;  // row scratch of the global-row tier
   size_t scr_words = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // per-launch profile of the current query (events around every kernel)
+  static constexpr int MAX_PROF = 16;
+  hipEvent_t pev[2 * MAX_PROF] = {nullptr};
+  const char* pname[MAX_PROF] = {nullptr};
+  int nprof = 0;
+  bool prof_on = true;
   std::mutex mu;
 };
 
@@ -43,6 +52,10 @@ struct jgpm_graph_s {
   std::vector<int32_t> tier_h[jk::NUM_TIERS];  // root ids per block tier (ascending)
   int32_t* tier_d[jk::NUM_TIERS] = {nullptr};
   std::vector<int32_t> huge_h;                 // roots beyond the largest tier
+  std::vector<int64_t> sym_rp_h;               // host copy of the symmetric row pointers
+  int32_t* d_src = nullptr;                    // COO source per symmetric edge (lazy)
+  int32_t* d_tri = nullptr;                    // triangles per symmetric edge (lazy)
+  int64_t* d_nsum = nullptr;                   // neighbour-degree sums (lazy)
   int64_t bytes = 0;
 };
 
@@ -50,7 +63,13 @@ namespace jgpm {
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
 int ensure_scratch(jgpm_ctx_s* c, size_t words);
 int clique_k(int pattern);
-int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st);
+int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
+                   bool split_slots = false);
+// profile hooks around one kernel launch on the context stream
+void prof_begin(jgpm_ctx_s* c, hipStream_t st, const char* name);
+void prof_end(jgpm_ctx_s* c, hipStream_t st);
+int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int out_slot, hipStream_t st);
+int enqueue_vertex_sum(jgpm_graph_s* g, int mode, int64_t rb, int64_t re, int out_slot, hipStream_t st);
 // Enqueue every kernel of `pattern` for roots [rb, re); results land in
 // accumulator slots base_slot.. (see finish_pattern).
 int enqueue_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re, int base_slot, hipStream_t st);

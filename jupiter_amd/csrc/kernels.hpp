@@ -30,6 +30,31 @@ struct CliqueArgs {
 };
 
 hipError_t launch_clique(int k, bool bytes, int tier, const CliqueArgs& a, int grid, hipStream_t st);
+
+// Symmetric-graph patterns P1..P12: one wave per directed edge of [e0, e0+ne).
+struct SymArgs {
+  const int64_t* rp;     // symmetric row pointers
+  const int32_t* ci;     // symmetric column indices
+  const int32_t* src;    // COO source of every directed edge
+  const int32_t* tri;    // per directed edge |N(u) ∩ N(v)| (patterns that use it)
+  const int64_t* nsum;   // per vertex sum of neighbour degrees (P3)
+  int64_t e0, ne;        // edge range
+  unsigned long long* qctr;
+  int32_t chunk;         // edges per queue grab
+  int32_t* scr;          // per-wave list scratch for lists longer than the LDS capacity
+  int64_t scr_stride;    // ints per scratch list (>= max degree)
+  uint64_t* out;
+};
+hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
+int sym_waves_per_block();
+int sym_lists_per_wave();
+hipError_t launch_aux_src(const int64_t* rp, int32_t nv, int32_t* src, int grid, hipStream_t st);
+hipError_t launch_aux_tri(const int64_t* rp, const int32_t* ci, const int32_t* src, int64_t ne, int32_t* tri,
+                          unsigned long long* qctr, int grid, hipStream_t st);
+hipError_t launch_aux_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int64_t* nsum, int grid, hipStream_t st);
+// mode 0: sum C(d,3) (3-stars); mode 1: sum t_v (d_v - 2) (tailed triangles)
+hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
+                              unsigned long long* out, int grid, hipStream_t st);
 int tier_max_deg(int tier);
 size_t tier_scratch_words(int tier);
 

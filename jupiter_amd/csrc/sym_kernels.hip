@@ -1,0 +1,484 @@
+// sym_kernels.hip — gfx950 kernels for the patterns P1..P12 on the symmetric
+// CSR, plus the per-edge triangle counts and per-vertex sums they use.
+//
+// Replaces P1_kernel..P12_kernel (src/message_passing_gpu_kernels.cuh:104-1109)
+// and the delegation producer/extend pairs of src/delegation_gpu_kernels.cuh.
+// One wave owns one directed edge (v0, v1) of the root range, exactly the
+// edge loop of the reference (eid over the local COO list, :22 etc.), and adds
+// that edge's contribution to the count, so any root range gives the same
+// partial count as the reference's per-GPU edge range.
+//
+// The contribution of an edge is the reference nest's value, but every
+// innermost "for x in L: if x not excluded: count++" loop is replaced by an
+// exact closed form in set sizes (|L| minus the excluded members that lie in
+// L), and several nests are folded into sums over S = N(v0) ∩ N(v1) of
+// per-edge triangle counts t(u,w) = |N(u) ∩ N(w)| and per-triangle common
+// neighbour counts q(v2) = |S ∩ N(v2)|:
+//   P2  C(s,2)                                   s = |S|         (v1 < v0)
+//   P3  s'(D0 - d1 - d0 + 1 - s) - sum_{S'}(d(v2) - 2) - sum_{S'}(t(v0,v2) - 1)
+//       with S' = {v2 in S : v2 > v1}, D0 = sum_{u in N(v0)} d(u)
+//   P4  s*A - sum_S (t(v0,v3) - 1) - sum_S (t(v1,v2) - 1),
+//       A = sum_{v3 in N(v1)\{v0}} (|N(v0) ∩ N(v3)| - 1)          (v1 < v0)
+//   P5  (s - 2) * (sum_S q(v2)) / 2                               (v1 < v0)
+//   P6  sum over pairs of S of (|N(v2) ∩ N(v3)| - 2)              (v1 < v0)
+//   P7  sum_{S'} ((s - 1)(t(v0,v2) - 1) - q(v2))
+//   P8  nest kept: v2 in S, v2 < v1; v3 in N(v0) ∩ N(v2), v3 < v1; |{x in S ∩ N(v3): x < v2}|
+//   P9  sum_{S'} C(q(v2), 2)                                      (v0 < v1)
+//   P10 v2 in S, v3 in N(v0) ∩ N(v2), v3 > v1:
+//       (s - 1 - [v3 in S])(t(v0,v3) - 1 - [v3 in S]) - (|S ∩ N(v3)| - 1)
+//   P11 sum_{S'} (a b c - q (a + b + c) + 2 q), a = s-1, b = t(v0,v2)-1, c = t(v1,v2)-1   (v0 < v1)
+//   P12 (s - 3) sum_S C(q(v2), 2)                                 (v0 < v1)
+// P1 keeps the reference nest (v2 in N(v0), v2 < v1; |N(v1) ∩ N(v2) ∩ [0, v0)|).
+//
+// Set operations are wave-wide: the shorter list is the lookup side, each
+// lane binary-searches its element in the longer list (staged in LDS when it
+// fits, the galloping/binary-search side of the merge/gallop choice),
+// ballot + mbcnt compact the matches in order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace jk {
+
+namespace {
+
+constexpr int WPB = 4;        // waves per block, each an independent edge worker
+constexpr int CAP = 256;      // LDS list capacity per wave (ints)
+constexpr int STAGE = 256;    // longest search list staged into LDS
+constexpr int NLIST = 5;      // per-wave lists: S, posS0, posS1, L, posL0
+
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ int64_t wsum64(int64_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)((uint64_t)v >> 32);
+    lo = __shfl_xor(lo, m);
+    hi = __shfl_xor(hi, m);
+    v += (int64_t)(((uint64_t)hi << 32) | lo);
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wsum(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ int lb(const int32_t* a, int n, int32_t y) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < y) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+struct Wave {
+  int32_t* stage;  // LDS, STAGE ints
+};
+
+// |A ∩ B| restricted to values < ub.  All lanes call with uniform arguments.
+__device__ int64_t w_icount(Wave& w, const int32_t* A, int na, const int32_t* B, int nb, int32_t ub) {
+  if (ub != INT32_MAX) {
+    na = lb(A, na, ub);
+    nb = lb(B, nb, ub);
+  }
+  if (na > nb) {
+    const int32_t* t = A; A = B; B = t;
+    int u = na; na = nb; nb = u;
+  }
+  if (na == 0) return 0;
+  const int32_t* S = B;
+  if (nb <= STAGE && na > 4) {
+    for (int k = lane(); k < nb; k += 64) w.stage[k] = B[k];
+    __syncwarp();
+    S = w.stage;
+  }
+  int c = 0;
+  for (int i = lane(); i < na; i += 64) {
+    int32_t x = A[i];
+    int p = lb(S, nb, x);
+    c += (p < nb && S[p] == x);
+  }
+  c = wsum(c);
+  __syncwarp();
+  return c;
+}
+
+// out = A ∩ B (ascending, values < ub); posA/posB (optional) receive the
+// element's index in A and B.  Returns the size.
+__device__ int w_imat(Wave& w, const int32_t* A, int na, const int32_t* B, int nb, int32_t ub, int32_t* out,
+                      int32_t* posA, int32_t* posB) {
+  if (ub != INT32_MAX) {
+    na = lb(A, na, ub);
+    nb = lb(B, nb, ub);
+  }
+  const bool sw = na > nb;
+  const int32_t* Lk = sw ? B : A;
+  const int32_t* Sr = sw ? A : B;
+  const int nl = sw ? nb : na;
+  const int ns = sw ? na : nb;
+  if (nl == 0) return 0;
+  const int32_t* S = Sr;
+  if (ns <= STAGE && nl > 4) {
+    for (int k = lane(); k < ns; k += 64) w.stage[k] = Sr[k];
+    __syncwarp();
+    S = w.stage;
+  }
+  int n = 0;
+  for (int base = 0; base < nl; base += 64) {
+    const int i = base + lane();
+    bool f = false;
+    int32_t x = 0;
+    int p = 0;
+    if (i < nl) {
+      x = Lk[i];
+      p = lb(S, ns, x);
+      f = p < ns && S[p] == x;
+    }
+    uint64_t m = __ballot(f);
+    if (f) {
+      int k = n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      out[k] = x;
+      if (posA) posA[k] = sw ? p : i;
+      if (posB) posB[k] = sw ? i : p;
+    }
+    n += __popcll(m);
+  }
+  __syncwarp();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  return n;
+}
+
+__device__ __forceinline__ bool in_sorted(const int32_t* a, int n, int32_t x) {
+  int p = lb(a, n, x);
+  return p < n && a[p] == x;
+}
+
+struct Ctx {
+  const int64_t* rp;
+  const int32_t* ci;
+  const int32_t* tri;
+  const int64_t* nsum;
+  __device__ const int32_t* N(int32_t v) const { return ci + rp[v]; }
+  __device__ int d(int32_t v) const { return (int)(rp[v + 1] - rp[v]); }
+};
+
+__device__ __forceinline__ int64_t C2(int64_t x) { return x >= 2 ? x * (x - 1) / 2 : 0; }
+
+// Contribution of directed edge (v0, v1) to pattern P (wave-uniform value).
+template <int P>
+__device__ int64_t edge_value(const Ctx& g, Wave& w, int32_t v0, int32_t v1, int32_t** L) {
+  const int32_t* N0 = g.N(v0);
+  const int32_t* N1 = g.N(v1);
+  const int d0 = g.d(v0), d1 = g.d(v1);
+  int32_t* S = L[0];
+  int32_t* p0 = L[1];
+  int32_t* p1 = L[2];
+  int32_t* T = L[3];
+  int32_t* pT = L[4];
+  const int ln = lane();
+  if constexpr (P == 1) {
+    if (v1 >= v0) return 0;
+    int64_t c = 0;
+    for (int j = 0; j < d0; ++j) {
+      int32_t v2 = N0[j];
+      if (v2 >= v1) break;
+      c += w_icount(w, N1, d1, g.N(v2), g.d(v2), v0);
+    }
+    return c;
+  } else if constexpr (P == 2) {
+    if (v1 >= v0) return 0;
+    return C2(w_icount(w, N0, d0, N1, d1, INT32_MAX));
+  } else if constexpr (P == 3) {
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, nullptr);
+    int64_t sp = 0, sd = 0, st = 0;
+    for (int k = ln; k < n; k += 64) {
+      int32_t v2 = S[k];
+      if (v2 > v1) {
+        sp += 1;
+        sd += g.d(v2) - 2;
+        st += g.tri[g.rp[v0] + p0[k]] - 1;
+      }
+    }
+    sp = wsum64(sp);
+    sd = wsum64(sd);
+    st = wsum64(st);
+    return sp * (g.nsum[v0] - d1 - d0 + 1 - n) - sd - st;
+  } else if constexpr (P == 4) {
+    if (v1 >= v0) return 0;
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, p1);
+    if (n == 0) return 0;
+    int64_t A = 0;
+    for (int j = 0; j < d1; ++j) {
+      int32_t v3 = N1[j];
+      if (v3 == v0) continue;
+      A += w_icount(w, N0, d0, g.N(v3), g.d(v3), INT32_MAX) - 1;
+    }
+    int64_t s0 = 0, s1 = 0;
+    for (int k = ln; k < n; k += 64) {
+      s0 += g.tri[g.rp[v0] + p0[k]] - 1;
+      s1 += g.tri[g.rp[v1] + p1[k]] - 1;
+    }
+    return (int64_t)n * A - wsum64(s0) - wsum64(s1);
+  } else if constexpr (P == 5) {
+    if (v1 >= v0) return 0;
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
+    if (n < 3) return 0;
+    int64_t k2 = 0;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      k2 += w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX);
+    }
+    return (int64_t)(n - 2) * (k2 / 2);
+  } else if constexpr (P == 6) {
+    if (v1 >= v0) return 0;
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
+    int64_t c = 0;
+    for (int i = 1; i < n; ++i) {
+      int32_t v2 = S[i];
+      for (int j = 0; j < i; ++j) {
+        int32_t v3 = S[j];
+        c += w_icount(w, g.N(v2), g.d(v2), g.N(v3), g.d(v3), INT32_MAX) - 2;
+      }
+    }
+    return c;
+  } else if constexpr (P == 7) {
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, nullptr);
+    int64_t c = 0;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      if (v2 <= v1) continue;
+      int64_t q = w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX);
+      c += (int64_t)(n - 1) * (g.tri[g.rp[v0] + p0[j]] - 1) - q;
+    }
+    return c;
+  } else if constexpr (P == 8) {
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
+    int64_t c = 0;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      if (v2 >= v1) break;  // S ascending
+      const int m = w_imat(w, N0, d0, g.N(v2), g.d(v2), v1, T, nullptr, nullptr);
+      for (int k = 0; k < m; ++k) {
+        int32_t v3 = T[k];
+        c += w_icount(w, S, n, g.N(v3), g.d(v3), v2);
+      }
+    }
+    return c;
+  } else if constexpr (P == 9) {
+    if (v1 <= v0) return 0;
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
+    int64_t c = 0;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      if (v2 <= v1) continue;
+      c += C2(w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX));
+    }
+    return c;
+  } else if constexpr (P == 10) {
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
+    int64_t c = 0;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      const int m = w_imat(w, N0, d0, g.N(v2), g.d(v2), INT32_MAX, T, pT, nullptr);
+      for (int k = 0; k < m; ++k) {
+        int32_t v3 = T[k];
+        if (v3 <= v1) continue;
+        const int64_t ins = in_sorted(S, n, v3) ? 1 : 0;
+        const int64_t t03 = g.tri[g.rp[v0] + pT[k]];
+        const int64_t q3 = w_icount(w, S, n, g.N(v3), g.d(v3), INT32_MAX);
+        c += (n - 1 - ins) * (t03 - 1 - ins) - (q3 - 1);
+      }
+    }
+    return c;
+  } else if constexpr (P == 11) {
+    if (v1 <= v0) return 0;
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, p1);
+    int64_t c = 0;
+    const int64_t a = n - 1;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      if (v2 <= v1) continue;
+      const int64_t q = w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX);
+      const int64_t b = g.tri[g.rp[v0] + p0[j]] - 1;
+      const int64_t cc = g.tri[g.rp[v1] + p1[j]] - 1;
+      c += a * b * cc - q * (a + b + cc) + 2 * q;
+    }
+    return c;
+  } else if constexpr (P == 12) {
+    if (v1 <= v0) return 0;
+    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
+    if (n < 3) return 0;
+    int64_t c = 0;
+    for (int j = 0; j < n; ++j) {
+      int32_t v2 = S[j];
+      c += C2(w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX));
+    }
+    return (int64_t)(n - 3) * c;
+  }
+  return 0;
+}
+
+}  // namespace
+
+template <int P>
+__global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
+  __shared__ int32_t s_list[WPB][NLIST][CAP];
+  __shared__ int32_t s_stage[WPB][STAGE];
+  __shared__ uint64_t s_part[WPB];
+  const int wv = threadIdx.x >> 6;
+  const int ln = lane();
+  Wave w{s_stage[wv]};
+  Ctx g{a.rp, a.ci, a.tri, a.nsum};
+  int32_t* glist = a.scr + ((int64_t)blockIdx.x * WPB + wv) * NLIST * a.scr_stride;
+  uint64_t acc = 0;
+  for (;;) {
+    int64_t t = 0;
+    if (ln == 0) t = (int64_t)atomicAdd((unsigned long long*)a.qctr, (unsigned long long)a.chunk);
+    const int64_t base = __shfl(t, 0);
+    if (base >= a.ne) break;
+    const int64_t end = min(a.ne, base + (int64_t)a.chunk);
+    for (int64_t e = base; e < end; ++e) {
+      const int64_t eid = a.e0 + e;
+      const int32_t v0 = a.src[eid];
+      const int32_t v1 = a.ci[eid];
+      // lists fit in LDS when the smaller endpoint degree does
+      const int dmin = min((int)(a.rp[v0 + 1] - a.rp[v0]), (int)(a.rp[v1 + 1] - a.rp[v1]));
+      int32_t* L[NLIST];
+      const bool small = dmin <= CAP && (P != 8 && P != 10 ? true : (int)(a.rp[v0 + 1] - a.rp[v0]) <= CAP);
+#pragma unroll
+      for (int k = 0; k < NLIST; ++k) L[k] = small ? s_list[wv][k] : glist + (int64_t)k * a.scr_stride;
+      int64_t v = edge_value<P>(g, w, v0, v1, L);
+      if (ln == 0) acc += (uint64_t)v;
+    }
+  }
+  if (ln == 0) s_part[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+    for (int k = 0; k < WPB; ++k) s += s_part[k];
+    if (s) atomicAdd((unsigned long long*)a.out, (unsigned long long)s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Auxiliary per-graph arrays.
+
+// src[e] = source vertex of directed edge e (the reference's COO src list,
+// src/jupiter_gpm.cu:129-146); one wave per vertex.
+__global__ void k_fill_src(const int64_t* rp, int32_t nv, int32_t* src) {
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wid; v < nv; v += nw)
+    for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) src[e] = (int32_t)v;
+}
+
+// tri[e] = |N(u) ∩ N(v)| for every directed edge e = (u, v): one wave per edge.
+__global__ void __launch_bounds__(256) k_edge_tri(const int64_t* rp, const int32_t* ci, const int32_t* src,
+                                                  int64_t ne, int32_t* tri, unsigned long long* qctr) {
+  __shared__ int32_t s_stage[4][STAGE];
+  Wave w{s_stage[threadIdx.x >> 6]};
+  const int ln = lane();
+  for (;;) {
+    int64_t t = 0;
+    if (ln == 0) t = (int64_t)atomicAdd(qctr, 64ull);
+    const int64_t base = __shfl(t, 0);
+    if (base >= ne) break;
+    const int64_t end = min(ne, base + 64);
+    for (int64_t e = base; e < end; ++e) {
+      const int32_t u = src[e], v = ci[e];
+      int64_t c = w_icount(w, ci + rp[u], (int)(rp[u + 1] - rp[u]), ci + rp[v], (int)(rp[v + 1] - rp[v]), INT32_MAX);
+      if (ln == 0) tri[e] = (int32_t)c;
+    }
+  }
+}
+
+// nsum[v] = sum of the degrees of v's neighbours.
+__global__ void k_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int64_t* nsum) {
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wid; v < nv; v += nw) {
+    int64_t s = 0;
+    for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) {
+      int32_t u = ci[e];
+      s += rp[u + 1] - rp[u];
+    }
+    s = wsum64(s);
+    if (lane() == 0) nsum[v] = s;
+  }
+}
+
+// Per-vertex pattern sums over the root range: 3-stars C(d,3) and tailed
+// triangles t_v (d_v - 2) with t_v = (sum of tri over v's edges) / 2.
+__global__ void k_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
+                              unsigned long long* out) {
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  uint64_t acc = 0;
+  for (int64_t v = rb + wid; v < re; v += nw) {
+    const int64_t d = rp[v + 1] - rp[v];
+    if (mode == 0) {
+      if (lane() == 0 && d >= 3) acc += (uint64_t)(d * (d - 1) / 2) * (uint64_t)(d - 2) / 3;
+    } else {
+      int64_t s = 0;
+      for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) s += tri[e];
+      s = wsum64(s);
+      if (lane() == 0 && d >= 2) acc += (uint64_t)(s / 2) * (uint64_t)(d - 2);
+    }
+  }
+  acc = (uint64_t)wsum64((int64_t)acc);
+  if (lane() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+// ---------------------------------------------------------------------------
+template <int P>
+static hipError_t launch_p(const SymArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(sym_kernel<P>, dim3(grid), dim3(64 * WPB), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st) {
+  switch (p) {
+    case 1: return launch_p<1>(a, grid, st);
+    case 2: return launch_p<2>(a, grid, st);
+    case 3: return launch_p<3>(a, grid, st);
+    case 4: return launch_p<4>(a, grid, st);
+    case 5: return launch_p<5>(a, grid, st);
+    case 6: return launch_p<6>(a, grid, st);
+    case 7: return launch_p<7>(a, grid, st);
+    case 8: return launch_p<8>(a, grid, st);
+    case 9: return launch_p<9>(a, grid, st);
+    case 10: return launch_p<10>(a, grid, st);
+    case 11: return launch_p<11>(a, grid, st);
+    case 12: return launch_p<12>(a, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+int sym_waves_per_block() { return WPB; }
+int sym_lists_per_wave() { return NLIST; }
+
+hipError_t launch_aux_src(const int64_t* rp, int32_t nv, int32_t* src, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_src, dim3(grid), dim3(256), 0, st, rp, nv, src);
+  return hipGetLastError();
+}
+hipError_t launch_aux_tri(const int64_t* rp, const int32_t* ci, const int32_t* src, int64_t ne, int32_t* tri,
+                          unsigned long long* qctr, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_edge_tri, dim3(grid), dim3(256), 0, st, rp, ci, src, ne, tri, qctr);
+  return hipGetLastError();
+}
+hipError_t launch_aux_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int64_t* nsum, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_nsum, dim3(grid), dim3(256), 0, st, rp, ci, nv, nsum);
+  return hipGetLastError();
+}
+hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
+                              unsigned long long* out, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_vertex_sums, dim3(grid), dim3(256), 0, st, rp, tri, rb, re, mode, out);
+  return hipGetLastError();
+}
+
+}  // namespace jk
