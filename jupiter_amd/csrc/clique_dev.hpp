@@ -1,0 +1,490 @@
+// clique_dev.hpp — device routines of the clique engine (bitmap build,
+// clique recursion), shared by the single-query kernels and the batched /
+// serving kernel.  See clique_kernels.hip for the design.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace jk {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl_xor(lo, m);
+  hi = __shfl_xor(hi, m);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
+  return v;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int l = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (l >= o) x += y;
+  }
+  return x;
+}
+
+// last index i in [0, n) with a[i] <= f, given a[0] <= f < a[n]
+__device__ __forceinline__ int seg_search(const int* a, int n, int f) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] <= f) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// first index p in [0, n) with a[p] >= y (n if none)
+template <class T>
+__device__ __forceinline__ int lower_bound_dev(const T* a, int n, int y) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < y) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool bsearch_global(const int32_t* __restrict__ a, int64_t n, int32_t y) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    int32_t v = a[mid];
+    if (v < y) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && a[lo] == y;
+}
+
+// Group of G waves sharing one root (G == 1: a lone wave; G > 1: the block).
+template <int G>
+struct Grp {
+  static constexpr int size = 64 * G;
+  __device__ static int rank() { return G == 1 ? lane_id() : (int)threadIdx.x; }
+  __device__ static int wave() { return G == 1 ? 0 : (int)(threadIdx.x >> 6); }
+  __device__ static void sync() {
+    if constexpr (G == 1) __syncwarp(); else __syncthreads();
+  }
+};
+
+// Exclusive scan of a[0..n) in place (n <= 64 * G * per-thread chunk), sets a[n] = total.
+template <int G>
+__device__ void group_excl_scan(int* a, int n, int* wtot) {
+  const int r = Grp<G>::rank();
+  const int per = (n + Grp<G>::size - 1) / Grp<G>::size;
+  const int b = r * per;
+  const int e = min(n, b + per);
+  int s = 0;
+  for (int t = b; t < e; ++t) s += a[t];
+  int incl = wave_incl_scan(s);
+  int base = 0;
+  if constexpr (G > 1) {
+    const int w = Grp<G>::wave();
+    if (lane_id() == 63) wtot[w] = incl;
+    __syncthreads();
+    for (int k = 0; k < w; ++k) base += wtot[k];
+    if (w == G - 1 && lane_id() == 63) wtot[G] = base + incl;
+  }
+  int run = base + incl - s;
+  for (int t = b; t < e; ++t) {
+    int v = a[t];
+    a[t] = run;
+    run += v;
+  }
+  if constexpr (G == 1) {
+    if (lane_id() == 63) a[n] = incl;
+  }
+  Grp<G>::sync();
+  if constexpr (G > 1) {
+    if (threadIdx.x == 0) a[n] = wtot[G];
+    __syncthreads();
+  }
+}
+
+// Count (depth)-cliques-with-prefix over single-word rows `R`: depth 1 is
+// popcount(C); deeper levels branch on every member x of C with C & R[x].
+// With BYTES the weights w[x] of every visited member are summed instead
+// (work-model bytes of the reference nest).
+template <int DEPTH, bool BYTES, class IDX = int>
+__device__ uint64_t rec1(uint64_t C, const uint64_t* R, const uint32_t* w, const IDX* map = nullptr) {
+  if constexpr (DEPTH <= 0) {
+    return 0;
+  } else if constexpr (DEPTH == 1 && !BYTES) {
+    return (uint64_t)__popcll(C);
+  } else {
+    uint64_t s = 0;
+    if constexpr (!BYTES) {
+      if (__popcll(C) < DEPTH) return 0;
+    }
+    // Positions follow vertex ids, not the DAG order, so a successor of x
+    // may sit at a lower position: branch on C & R[x] with the whole of C.
+    const uint64_t C0 = C;
+    while (C) {
+      int x = __builtin_ctzll(C);
+      C &= C - 1;
+      if constexpr (BYTES) s += w[map ? (int)map[x] : x];
+      if constexpr (DEPTH > 1) s += rec1<DEPTH - 1, BYTES, IDX>(C0 & R[x], R, w, map);
+    }
+    return s;
+  }
+}
+
+// Multi-word fallback: words of the current set are the AND of the rows of
+// the chosen positions p[0..NP), recomputed on the fly (no register arrays).
+template <int DEPTH, int NP, bool BYTES, class RowT>
+__device__ uint64_t recW(const RowT& R, int wlo, int whi, const int* p, const uint32_t* wt) {
+  uint64_t s = 0;
+  for (int w = wlo; w <= whi; ++w) {
+    uint64_t m = R(p[0], w);
+#pragma unroll
+    for (int k = 1; k < NP; ++k) m &= R(p[k], w);
+    if constexpr (DEPTH == 1 && !BYTES) {
+      s += (uint64_t)__popcll(m);
+    } else {
+      while (m) {
+        int x = (w << 6) | __builtin_ctzll(m);
+        m &= m - 1;
+        if constexpr (BYTES) s += wt[x];
+        if constexpr (DEPTH > 1) {
+          int q[NP + 1];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) q[k] = p[k];
+          q[NP] = x;
+          s += recW<DEPTH - 1, NP + 1, BYTES>(R, wlo, whi, q, wt);
+        }
+      }
+    }
+  }
+  return s;
+}
+
+struct RowsLDS {
+  uint64_t* p;
+  int W;
+  __device__ uint64_t operator()(int i, int w) const { return p[i * W + w]; }
+};
+struct RowsGlobal {
+  uint64_t* p;
+  int W;
+  __device__ uint64_t operator()(int i, int w) const { return __builtin_nontemporal_load(p + (int64_t)i * W + w); }
+};
+
+// Block reduction of a per-thread uint64 and one atomicAdd per block.
+template <int NT>
+__device__ void block_add(uint64_t v, uint64_t* out) {
+  __shared__ uint64_t part[NT / 64];
+  v = wave_sum_u64(v);
+  if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; ++k) s += part[k];
+    if (s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+  }
+}
+
+// Next set bit at or after `from` in the first CW words of c (-1 if none).
+template <int CWMAX>
+__device__ __forceinline__ int next_bit(const uint64_t (&c)[CWMAX], int CW, int from) {
+#pragma unroll
+  for (int w = 0; w < CWMAX; ++w) {
+    if (w < CW && from < (w + 1) * 64) {
+      uint64_t m = c[w];
+      if (from > w * 64) m &= (~0ull) << (from - w * 64);
+      if (m) return w * 64 + __builtin_ctzll(m);
+    }
+  }
+  return -1;
+}
+
+// Clique recursion over compressed multi-word rows C (CW words per row):
+// the candidate set lives in registers, rows in LDS.
+template <int DEPTH, int CWMAX, bool BYTES>
+__device__ uint64_t recC(const uint64_t (&c)[CWMAX], int CW, const uint64_t* C, const uint32_t* wt) {
+  if constexpr (DEPTH <= 0) {
+    return 0;
+  } else if constexpr (DEPTH == 1 && !BYTES) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int w = 0; w < CWMAX; ++w)
+      if (w < CW) s += (uint64_t)__popcll(c[w]);
+    return s;
+  } else {
+    uint64_t s = 0;
+    if constexpr (!BYTES) {
+      int pc = 0;
+#pragma unroll
+      for (int w = 0; w < CWMAX; ++w)
+        if (w < CW) pc += __popcll(c[w]);
+      if (pc < DEPTH) return 0;
+    }
+    for (int x = next_bit<CWMAX>(c, CW, 0); x >= 0; x = next_bit<CWMAX>(c, CW, x + 1)) {
+      if constexpr (BYTES) s += wt[x];
+      if constexpr (DEPTH > 1) {
+        uint64_t nx[CWMAX];
+#pragma unroll
+        for (int w = 0; w < CWMAX; ++w) nx[w] = (w < CW) ? (c[w] & C[x * CW + w]) : 0ull;
+        s += recC<DEPTH - 1, CWMAX, BYTES>(nx, CW, C, wt);
+      }
+    }
+    return s;
+  }
+}
+
+// Clique recursion over the original multi-word rows restricted to the
+// word window [wlo, wlo + nw) of the first row: the candidate set lives in
+// WM registers, each branch reads nw words of one row.
+template <int DEPTH, int WM, bool BYTES, class RowT>
+__device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R, const uint32_t* wt) {
+  if constexpr (DEPTH <= 0) {
+    return 0;
+  } else if constexpr (DEPTH == 1 && !BYTES) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int w = 0; w < WM; ++w)
+      if (w < nw) s += (uint64_t)__popcll(c[w]);
+    return s;
+  } else {
+    uint64_t s = 0;
+    if constexpr (!BYTES) {
+      int pc = 0;
+#pragma unroll
+      for (int w = 0; w < WM; ++w)
+        if (w < nw) pc += __popcll(c[w]);
+      if (pc < DEPTH) return 0;
+    }
+    for (int b = next_bit<WM>(c, nw, 0); b >= 0; b = next_bit<WM>(c, nw, b + 1)) {
+      const int x = wlo * 64 + b;
+      if constexpr (BYTES) s += wt[x];
+      if constexpr (DEPTH > 1) {
+        uint64_t nx[WM];
+#pragma unroll
+        for (int w = 0; w < WM; ++w) nx[w] = (w < nw) ? (c[w] & R(x, wlo + w)) : 0ull;
+        s += recR<DEPTH - 1, WM, BYTES>(nx, wlo, nw, R, wt);
+      }
+    }
+    return s;
+  }
+}
+
+// LDS layout of one root slot.  The build-phase arrays (segment offsets,
+// list bases, long-list degrees) and the count-phase arrays (compacted row
+// lists, deferred rows, the compressed bitmap) share one region.
+template <int G, int WMAX, int K, bool BYTES, bool LROWS>
+struct Lay {
+  static constexpr int DM = 64 * WMAX;
+  static constexpr bool ROWS = K >= 4;
+  static constexpr bool MULTI = WMAX > 1 && ROWS;
+  // compressed-row capacity of the block-cooperative path (rows of S > 64)
+  static constexpr int NC = 0;  // (no block-cooperative compressed rows)
+  static constexpr int CWMAX = NC > 0 ? NC / 64 : 1;
+  static constexpr size_t al(size_t x) { return (x + 15) & ~size_t(15); }
+  static constexpr size_t L_OFF = 0;
+  static constexpr size_t W_OFF = al(L_OFF + 4 * DM);
+  static constexpr size_t R_OFF = al(W_OFF + (BYTES ? 4 * DM : 0));
+  static constexpr size_t U_OFF = al(R_OFF + ((LROWS && ROWS) ? (size_t)8 * DM * WMAX : 0));
+  // build view
+  static constexpr size_t B_OFFS = 0;
+  static constexpr size_t B_UB = al(B_OFFS + 4 * (DM + 1));
+  static constexpr size_t B_HDU = al(B_UB + 8 * DM);
+  static constexpr size_t B_END = al(B_HDU + 4 * DM);
+  // count view
+  static constexpr size_t C_J = 0;                                          // int16 [G][DM]
+  static constexpr size_t C_JB = al(C_J + (MULTI ? 2 * G * DM : 0));        // (unused)
+  static constexpr size_t C_DEF = C_JB;
+  static constexpr size_t C_COMP = al(C_DEF + (MULTI ? 2 * DM : 0));        // u64 [NC * CWMAX]
+  static constexpr size_t C_SC = al(C_COMP + (size_t)8 * NC * CWMAX);       // u64 [G][64]
+  static constexpr size_t C_SCW = al(C_SC + (MULTI ? (size_t)8 * G * 64 : 0));  // u32 [G][64]
+  static constexpr size_t C_WCB = C_SCW;
+  static constexpr size_t C_END = al(C_WCB + ((MULTI && BYTES) ? (size_t)4 * NC : 0));
+  static constexpr size_t U_SIZE = B_END > C_END ? B_END : C_END;
+  static constexpr size_t SLOT = al(U_OFF + U_SIZE);
+};
+
+// Process one root v0 (list N+(v0) = ci[b0 .. b0 + d)) with a group of G
+// waves: build the bitmap rows in `rowp` and add the root's k-cliques (or,
+// with BYTES, its work-model bytes) to `acc`.  `ar` is the group's LDS
+// arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes.
+template <int G, int WMAX, int K, bool BYTES, bool LROWS>
+__device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, int64_t b0, int d,
+                             int ratio, unsigned char* ar, uint64_t* rowp, int32_t* s_wtot, uint64_t& acc) {
+  using LY = Lay<G, WMAX, K, BYTES, LROWS>;
+  constexpr int DM = LY::DM;
+  constexpr bool NEED_ROWS = LY::ROWS;
+  const int r = Grp<G>::rank();
+  const int wv = Grp<G>::wave();
+  const int ln = lane_id();
+  int32_t* L = reinterpret_cast<int32_t*>(ar + LY::L_OFF);
+  uint32_t* wL = reinterpret_cast<uint32_t*>(ar + LY::W_OFF);
+  unsigned char* U = ar + LY::U_OFF;
+  int32_t* off = reinterpret_cast<int32_t*>(U + LY::B_OFFS);
+  int64_t* ub = reinterpret_cast<int64_t*>(U + LY::B_UB);
+  int32_t* hdu = reinterpret_cast<int32_t*>(U + LY::B_HDU);
+  const int W = (d + 63) >> 6;
+  if constexpr (BYTES) {
+    if (r == 0) acc += 16 + 4 * (uint64_t)d;
+  }
+  // ---- stage root list and segment descriptors
+  for (int t = r; t < d; t += Grp<G>::size) L[t] = ci[b0 + t];
+  Grp<G>::sync();
+  for (int t = r; t < d; t += Grp<G>::size) {
+    int32_t u = L[t];
+    int64_t s = rp[u];
+    int du = (int)(rp[u + 1] - s);
+    bool heavy = du > 64 && du > ratio * d;
+    ub[t] = s;
+    off[t] = heavy ? 0 : du;
+    hdu[t] = heavy ? du : 0;
+    if constexpr (BYTES) {
+      wL[t] = 16u + 4u * (uint32_t)du;
+      acc += 16 + 4 * (uint64_t)du;  // f(v1) of the edge (v0, L[t])
+    }
+  }
+  if constexpr (NEED_ROWS) {
+    for (int t = r; t < d * W; t += Grp<G>::size) rowp[t] = 0;
+  }
+  if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  Grp<G>::sync();
+  if constexpr (K == 3 && BYTES) return;  // TC work model needs no build
+  group_excl_scan<G>(off, d, s_wtot);
+  const int T = off[d];
+  // ---- flattened stream of the light lists
+  for (int f = r; f < T; f += Grp<G>::size) {
+    int i = seg_search(off, d, f);
+    int32_t y = ci[ub[i] + (f - off[i])];
+    int p = lower_bound_dev(L, d, y);
+    if (p < d && L[p] == y) {
+      if constexpr (K == 3) {
+        ++acc;
+      } else {
+        atomicOr((unsigned long long*)&rowp[i * W + (p >> 6)], 1ull << (p & 63));
+      }
+    }
+  }
+  // ---- long lists: binary search every element of L in them
+  for (int i = 0; i < d; ++i) {
+    const int du = hdu[i];
+    if (du == 0) continue;
+    const int32_t* lst = ci + ub[i];
+    for (int t0 = wv * 64; t0 < d; t0 += Grp<G>::size) {
+      int t = t0 + ln;
+      bool f = t < d && bsearch_global(lst, du, L[t]);
+      uint64_t m = __ballot(f);
+      if constexpr (K == 3) {
+        if (ln == 0) acc += (uint64_t)__popcll(m);
+      } else {
+        if (ln == 0 && m) atomicOr((unsigned long long*)&rowp[i * W + (t0 >> 6)], m);
+      }
+    }
+  }
+  if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  Grp<G>::sync();
+  if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if constexpr (K == 3) return;
+  // ---- count (the build-phase region U is now free)
+  if constexpr (!LY::MULTI) {
+    // one word per row, lane i owns row i
+    if (r < d) {
+      const uint64_t Ri = rowp[r];
+      if constexpr (BYTES) acc += rec1<K - 3, true>(Ri, rowp, wL);
+      else acc += rec1<K - 2, false>(Ri, rowp, wL);
+    }
+  } else {
+    auto R = [&](int i, int w) -> uint64_t {
+      if constexpr (LROWS) return rowp[i * W + w];
+      else return RowsGlobal{rowp, W}(i, w);
+    };
+    int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;
+    uint64_t* Rc = reinterpret_cast<uint64_t*>(U + LY::C_SC) + (size_t)wv * 64;
+    for (int i = wv; i < d; i += G) {
+      // population and nonzero word window of row i (W <= 32 words)
+      const uint64_t myword = ln < W ? R(i, ln) : 0ull;
+      const uint64_t nz = __ballot(myword != 0ull);
+      if (!nz) continue;
+      const int wlo = __builtin_ctzll(nz);
+      const int whi = 63 - __builtin_clzll(nz);
+      // compact S = row i into J (ascending positions)
+      int n = 0;
+      for (int w = wlo; w <= whi; ++w) {
+        const uint64_t word = R(i, w);
+        if ((word >> ln) & 1ull) {
+          uint64_t below = ln ? (word & ((~0ull) >> (64 - ln))) : 0ull;
+          J[n + __popcll(below)] = (int16_t)((w << 6) | ln);
+        }
+        n += __popcll(word);
+      }
+      __syncwarp();
+      if constexpr (K == 4) {
+        for (int t = ln; t < n; t += 64) {
+          const int j = J[t];
+          if constexpr (BYTES) {
+            acc += wL[j];
+          } else {
+            uint64_t c = 0;
+            for (int w = wlo; w <= whi; ++w) c += __popcll(R(i, w) & R(j, w));
+            acc += c;
+          }
+        }
+      } else if (n <= 64) {
+        // compressed |S| x |S| successor bitmap, one word per member
+        uint64_t myrow = 0;
+        const int js = ln < n ? J[ln] : 0;
+        int curw = -1;
+        uint64_t word = 0;
+        for (int t = 0; t < n; ++t) {
+          const int x = J[t];
+          if (ln < n) {
+            if ((x >> 6) != curw) {
+              curw = x >> 6;
+              word = R(js, curw);
+            }
+            myrow |= ((word >> (x & 63)) & 1ull) << t;
+          }
+        }
+        if (ln < n) Rc[ln] = myrow;
+        __syncwarp();
+        if (ln < n) {
+          // weights of compressed members go through J back to L positions
+          if constexpr (BYTES) acc += wL[js] + rec1<K - 4, true, int16_t>(myrow, Rc, wL, J);
+          else acc += rec1<K - 3, false>(myrow, Rc, wL);
+        }
+      } else {
+        // lanes over the members j of S; candidate set R[i] & R[j] held
+        // in a register window of the row's nonzero words
+        for (int t = ln; t < n; t += 64) {
+          const int j = J[t];
+          if constexpr (WMAX <= 16) {
+            const int nw = whi - wlo + 1;
+            uint64_t c[WMAX];
+#pragma unroll
+            for (int w = 0; w < WMAX; ++w) c[w] = (w < nw) ? (R(i, wlo + w) & R(j, wlo + w)) : 0ull;
+            if constexpr (BYTES) acc += wL[j] + recR<K - 4, WMAX, true>(c, wlo, nw, R, wL);
+            else acc += recR<K - 3, WMAX, false>(c, wlo, nw, R, wL);
+          } else {
+            int p2[2] = {i, j};
+            if constexpr (BYTES) {
+              acc += wL[j];
+              acc += recW<K - 4, 2, true>(RowsGlobal{rowp, W}, wlo, whi, p2, wL);
+            } else {
+              acc += recW<K - 3, 2, false>(RowsGlobal{rowp, W}, wlo, whi, p2, wL);
+            }
+          }
+        }
+      }
+      __syncwarp();
+    }
+  }
+  Grp<G>::sync();
+}
+
+}  // namespace jk

@@ -15,7 +15,7 @@ namespace jgpm {
 static int grid_for(jgpm_ctx_s* c, int per_cu) { return std::max(1, per_cu) * c->num_cu; }
 
 // Lazily built per-graph arrays of the symmetric kernels.
-static int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st) {
+int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st) {
   jgpm_ctx_s* c = g->ctx;
   const int64_t ne = g->sym.nnz;
   if (!g->d_src) {
@@ -41,14 +41,14 @@ static int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStr
   return JGPM_OK;
 }
 
-static bool uses_tri(int p) { return p == 3 || p == 4 || p == 7 || p == 10 || p == 11; }
+bool pattern_uses_tri(int p) { return p == 3 || p == 4 || p == 7 || p == 10 || p == 11; }
 
 int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStream_t st) {
   if (!g->has_sym) return JGPM_E_GRAPHKIND;
   jgpm_ctx_s* c = g->ctx;
   // The triangle array is shared by later queries; its counter slot is
   // reset per query by ctx_reset_counters before it is used here.
-  int rc = ensure_sym_aux(g, uses_tri(p), p == 3, st);
+  int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
   if (rc) return rc;
   const int wpb = jk::sym_waves_per_block();
   const int grid = grid_for(c, 2);

@@ -209,6 +209,10 @@ extern "C" int jgpm_shutdown(jgpm_ctx_t c) {
   if (c->d_q) (void)hipFree(c->d_q);
   if (c->d_q64) (void)hipFree(c->d_q64);
   if (c->d_sscr) (void)hipFree(c->d_sscr);
+  if (c->d_bq) (void)hipFree(c->d_bq);
+  if (c->d_bcs) (void)hipFree(c->d_bcs);
+  if (c->d_bcnt) (void)hipFree(c->d_bcnt);
+  if (c->d_bq32) (void)hipFree(c->d_bq32);
   (void)hipEventDestroy(c->e0);
   (void)hipEventDestroy(c->e1);
   for (int i = 0; i < 2 * jgpm_ctx_s::MAX_PROF; ++i) (void)hipEventDestroy(c->pev[i]);
@@ -339,23 +343,23 @@ int clique_k(int pattern) {
 
 // Enqueue the clique engine for roots [rb, re) accumulating into out_slot.
 static const char* const kTierName[jk::NUM_TIERS] = {"clique_wave64", "clique_lds128", "clique_lds256",
-                                                     "clique_lds512", "clique_glb2048"};
+                                                     "clique_lds512", "clique_lds1024", "clique_glb2048"};
 
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
-                   bool split_slots) {
+                   bool split_slots, int first_tier, uint64_t* out_ptr, unsigned* qctr_base) {
   jgpm_ctx_s* c = g->ctx;
   if (!g->huge_h.empty()) {
     auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
     if (lo != g->huge_h.end() && *lo < re) return JGPM_E_BUDGET;
   }
   const int maxgrid = std::max(1, c->tune.blocks_per_cu) * c->num_cu;
-  for (int tier = jk::NUM_TIERS - 1; tier >= 0; --tier) {
+  for (int tier = jk::NUM_TIERS - 1; tier >= first_tier; --tier) {
     jk::CliqueArgs a{};
     a.rp = g->dag.rp;
     a.ci = g->dag.ci;
     a.ratio = std::max(1, c->tune.gallop_ratio);
-    a.qctr = c->d_q + (out_slot * jk::NUM_TIERS + tier) % CTX_SLOTS;
-    a.out = c->d_out + out_slot + (split_slots ? tier : 0);
+    a.qctr = qctr_base ? qctr_base + tier : c->d_q + (out_slot * jk::NUM_TIERS + tier) % CTX_SLOTS;
+    a.out = out_ptr ? out_ptr : c->d_out + out_slot + (split_slots ? tier : 0);
     a.gscr = nullptr;
     a.dmax = jk::tier_max_deg(tier);
     a.dmin = tier == jk::TIER_WAVE ? (bytes ? 0 : k - 1) : jk::tier_max_deg(tier - 1) + 1;

@@ -33,6 +33,12 @@ struct jgpm_ctx_s {
   unsigned long long* d_q64 = nullptr;  // CTX_SLOTS 64-bit work-queue counters
   int32_t* d_sscr = nullptr;  // list scratch of the symmetric-pattern kernels
   size_t sscr_ints = 0;
+  // batched / served launches: query table, chunk prefix, per-query counts
+  void* d_bq = nullptr;
+  int64_t* d_bcs = nullptr;
+  uint64_t* d_bcnt = nullptr;
+  unsigned* d_bq32 = nullptr;  // queue counters of the follow-up tier launches
+  int bcap = 0;
   uint64_t* d_scr = nullptr;  // row scratch of the global-row tier
   size_t scr_words = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -64,7 +70,10 @@ int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
 int ensure_scratch(jgpm_ctx_s* c, size_t words);
 int clique_k(int pattern);
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
-                   bool split_slots = false);
+                   bool split_slots = false, int first_tier = 0, uint64_t* out_ptr = nullptr,
+                   unsigned* qctr_base = nullptr);
+int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st);
+bool pattern_uses_tri(int p);
 // profile hooks around one kernel launch on the context stream
 void prof_begin(jgpm_ctx_s* c, hipStream_t st, const char* name);
 void prof_end(jgpm_ctx_s* c, hipStream_t st);

@@ -11,8 +11,9 @@ enum {
   TIER_LDS2 = 1,   // d <= 128   block per root, rows in LDS
   TIER_LDS4 = 2,   // d <= 256
   TIER_LDS8 = 3,   // d <= 512
-  TIER_GLB32 = 4,  // d <= 2048  block per root, rows in a global scratch slice
-  NUM_TIERS = 5
+  TIER_LDS16 = 4,  // d <= 1024  (128 KB of rows in LDS, one block per CU)
+  TIER_GLB32 = 5,  // d <= 2048  block per root, rows in a global scratch slice
+  NUM_TIERS = 6
 };
 
 struct CliqueArgs {

@@ -38,292 +38,11 @@
 #include <stdint.h>
 
 #include "kernels.hpp"
+#include "sym_dev.hpp"
 
 namespace jk {
 
-namespace {
-
-constexpr int WPB = 4;        // waves per block, each an independent edge worker
-constexpr int CAP = 256;      // LDS list capacity per wave (ints)
-constexpr int STAGE = 256;    // longest search list staged into LDS
-constexpr int NLIST = 5;      // per-wave lists: S, posS0, posS1, L, posL0
-
-__device__ __forceinline__ int lane() { return (int)__lane_id(); }
-
-__device__ __forceinline__ int64_t wsum64(int64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)((uint64_t)v >> 32);
-    lo = __shfl_xor(lo, m);
-    hi = __shfl_xor(hi, m);
-    v += (int64_t)(((uint64_t)hi << 32) | lo);
-  }
-  return v;
-}
-
-__device__ __forceinline__ int wsum(int v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
-
-__device__ __forceinline__ int lb(const int32_t* a, int n, int32_t y) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (a[mid] < y) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-struct Wave {
-  int32_t* stage;  // LDS, STAGE ints
-};
-
-// |A ∩ B| restricted to values < ub.  All lanes call with uniform arguments.
-__device__ int64_t w_icount(Wave& w, const int32_t* A, int na, const int32_t* B, int nb, int32_t ub) {
-  if (ub != INT32_MAX) {
-    na = lb(A, na, ub);
-    nb = lb(B, nb, ub);
-  }
-  if (na > nb) {
-    const int32_t* t = A; A = B; B = t;
-    int u = na; na = nb; nb = u;
-  }
-  if (na == 0) return 0;
-  const int32_t* S = B;
-  if (nb <= STAGE && na > 4) {
-    for (int k = lane(); k < nb; k += 64) w.stage[k] = B[k];
-    __syncwarp();
-    S = w.stage;
-  }
-  int c = 0;
-  for (int i = lane(); i < na; i += 64) {
-    int32_t x = A[i];
-    int p = lb(S, nb, x);
-    c += (p < nb && S[p] == x);
-  }
-  c = wsum(c);
-  __syncwarp();
-  return c;
-}
-
-// out = A ∩ B (ascending, values < ub); posA/posB (optional) receive the
-// element's index in A and B.  Returns the size.
-__device__ int w_imat(Wave& w, const int32_t* A, int na, const int32_t* B, int nb, int32_t ub, int32_t* out,
-                      int32_t* posA, int32_t* posB) {
-  if (ub != INT32_MAX) {
-    na = lb(A, na, ub);
-    nb = lb(B, nb, ub);
-  }
-  const bool sw = na > nb;
-  const int32_t* Lk = sw ? B : A;
-  const int32_t* Sr = sw ? A : B;
-  const int nl = sw ? nb : na;
-  const int ns = sw ? na : nb;
-  if (nl == 0) return 0;
-  const int32_t* S = Sr;
-  if (ns <= STAGE && nl > 4) {
-    for (int k = lane(); k < ns; k += 64) w.stage[k] = Sr[k];
-    __syncwarp();
-    S = w.stage;
-  }
-  int n = 0;
-  for (int base = 0; base < nl; base += 64) {
-    const int i = base + lane();
-    bool f = false;
-    int32_t x = 0;
-    int p = 0;
-    if (i < nl) {
-      x = Lk[i];
-      p = lb(S, ns, x);
-      f = p < ns && S[p] == x;
-    }
-    uint64_t m = __ballot(f);
-    if (f) {
-      int k = n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      out[k] = x;
-      if (posA) posA[k] = sw ? p : i;
-      if (posB) posB[k] = sw ? i : p;
-    }
-    n += __popcll(m);
-  }
-  __syncwarp();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  return n;
-}
-
-__device__ __forceinline__ bool in_sorted(const int32_t* a, int n, int32_t x) {
-  int p = lb(a, n, x);
-  return p < n && a[p] == x;
-}
-
-struct Ctx {
-  const int64_t* rp;
-  const int32_t* ci;
-  const int32_t* tri;
-  const int64_t* nsum;
-  __device__ const int32_t* N(int32_t v) const { return ci + rp[v]; }
-  __device__ int d(int32_t v) const { return (int)(rp[v + 1] - rp[v]); }
-};
-
-__device__ __forceinline__ int64_t C2(int64_t x) { return x >= 2 ? x * (x - 1) / 2 : 0; }
-
-// Contribution of directed edge (v0, v1) to pattern P (wave-uniform value).
-template <int P>
-__device__ int64_t edge_value(const Ctx& g, Wave& w, int32_t v0, int32_t v1, int32_t** L) {
-  const int32_t* N0 = g.N(v0);
-  const int32_t* N1 = g.N(v1);
-  const int d0 = g.d(v0), d1 = g.d(v1);
-  int32_t* S = L[0];
-  int32_t* p0 = L[1];
-  int32_t* p1 = L[2];
-  int32_t* T = L[3];
-  int32_t* pT = L[4];
-  const int ln = lane();
-  if constexpr (P == 1) {
-    if (v1 >= v0) return 0;
-    int64_t c = 0;
-    for (int j = 0; j < d0; ++j) {
-      int32_t v2 = N0[j];
-      if (v2 >= v1) break;
-      c += w_icount(w, N1, d1, g.N(v2), g.d(v2), v0);
-    }
-    return c;
-  } else if constexpr (P == 2) {
-    if (v1 >= v0) return 0;
-    return C2(w_icount(w, N0, d0, N1, d1, INT32_MAX));
-  } else if constexpr (P == 3) {
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, nullptr);
-    int64_t sp = 0, sd = 0, st = 0;
-    for (int k = ln; k < n; k += 64) {
-      int32_t v2 = S[k];
-      if (v2 > v1) {
-        sp += 1;
-        sd += g.d(v2) - 2;
-        st += g.tri[g.rp[v0] + p0[k]] - 1;
-      }
-    }
-    sp = wsum64(sp);
-    sd = wsum64(sd);
-    st = wsum64(st);
-    return sp * (g.nsum[v0] - d1 - d0 + 1 - n) - sd - st;
-  } else if constexpr (P == 4) {
-    if (v1 >= v0) return 0;
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, p1);
-    if (n == 0) return 0;
-    int64_t A = 0;
-    for (int j = 0; j < d1; ++j) {
-      int32_t v3 = N1[j];
-      if (v3 == v0) continue;
-      A += w_icount(w, N0, d0, g.N(v3), g.d(v3), INT32_MAX) - 1;
-    }
-    int64_t s0 = 0, s1 = 0;
-    for (int k = ln; k < n; k += 64) {
-      s0 += g.tri[g.rp[v0] + p0[k]] - 1;
-      s1 += g.tri[g.rp[v1] + p1[k]] - 1;
-    }
-    return (int64_t)n * A - wsum64(s0) - wsum64(s1);
-  } else if constexpr (P == 5) {
-    if (v1 >= v0) return 0;
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
-    if (n < 3) return 0;
-    int64_t k2 = 0;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      k2 += w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX);
-    }
-    return (int64_t)(n - 2) * (k2 / 2);
-  } else if constexpr (P == 6) {
-    if (v1 >= v0) return 0;
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
-    int64_t c = 0;
-    for (int i = 1; i < n; ++i) {
-      int32_t v2 = S[i];
-      for (int j = 0; j < i; ++j) {
-        int32_t v3 = S[j];
-        c += w_icount(w, g.N(v2), g.d(v2), g.N(v3), g.d(v3), INT32_MAX) - 2;
-      }
-    }
-    return c;
-  } else if constexpr (P == 7) {
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, nullptr);
-    int64_t c = 0;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      if (v2 <= v1) continue;
-      int64_t q = w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX);
-      c += (int64_t)(n - 1) * (g.tri[g.rp[v0] + p0[j]] - 1) - q;
-    }
-    return c;
-  } else if constexpr (P == 8) {
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
-    int64_t c = 0;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      if (v2 >= v1) break;  // S ascending
-      const int m = w_imat(w, N0, d0, g.N(v2), g.d(v2), v1, T, nullptr, nullptr);
-      for (int k = 0; k < m; ++k) {
-        int32_t v3 = T[k];
-        c += w_icount(w, S, n, g.N(v3), g.d(v3), v2);
-      }
-    }
-    return c;
-  } else if constexpr (P == 9) {
-    if (v1 <= v0) return 0;
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
-    int64_t c = 0;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      if (v2 <= v1) continue;
-      c += C2(w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX));
-    }
-    return c;
-  } else if constexpr (P == 10) {
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
-    int64_t c = 0;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      const int m = w_imat(w, N0, d0, g.N(v2), g.d(v2), INT32_MAX, T, pT, nullptr);
-      for (int k = 0; k < m; ++k) {
-        int32_t v3 = T[k];
-        if (v3 <= v1) continue;
-        const int64_t ins = in_sorted(S, n, v3) ? 1 : 0;
-        const int64_t t03 = g.tri[g.rp[v0] + pT[k]];
-        const int64_t q3 = w_icount(w, S, n, g.N(v3), g.d(v3), INT32_MAX);
-        c += (n - 1 - ins) * (t03 - 1 - ins) - (q3 - 1);
-      }
-    }
-    return c;
-  } else if constexpr (P == 11) {
-    if (v1 <= v0) return 0;
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, p0, p1);
-    int64_t c = 0;
-    const int64_t a = n - 1;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      if (v2 <= v1) continue;
-      const int64_t q = w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX);
-      const int64_t b = g.tri[g.rp[v0] + p0[j]] - 1;
-      const int64_t cc = g.tri[g.rp[v1] + p1[j]] - 1;
-      c += a * b * cc - q * (a + b + cc) + 2 * q;
-    }
-    return c;
-  } else if constexpr (P == 12) {
-    if (v1 <= v0) return 0;
-    const int n = w_imat(w, N0, d0, N1, d1, INT32_MAX, S, nullptr, nullptr);
-    if (n < 3) return 0;
-    int64_t c = 0;
-    for (int j = 0; j < n; ++j) {
-      int32_t v2 = S[j];
-      c += C2(w_icount(w, S, n, g.N(v2), g.d(v2), INT32_MAX));
-    }
-    return (int64_t)(n - 3) * c;
-  }
-  return 0;
-}
-
-}  // namespace
+using namespace symd;
 
 template <int P>
 __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
