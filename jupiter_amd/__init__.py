@@ -357,3 +357,45 @@ class Graph:
         out = ctypes.c_uint64()
         _check(lib().jgpm_work_bytes(self._h, pattern_id(pattern), rb, re, ctypes.byref(out)), "work_bytes")
         return int(out.value)
+
+
+class Server:
+    """Continuous-batching server over graphs resident on one context
+    (jgpm_serve_*).  submit() returns a ticket; wait()/poll() give
+    (count, latency_us)."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=0, slice_roots=0):
+        cfg = _ServeCfg(hbm_budget_bytes, max_inflight, slice_roots)
+        self._h = ctypes.c_void_p()
+        _check(lib().jgpm_serve_start(ctx._h, ctypes.byref(cfg), ctypes.byref(self._h)), "serve_start")
+
+    def submit(self, graph, pattern, roots=None):
+        rb, re = (0, -1) if roots is None else roots
+        t = ctypes.c_uint64()
+        _check(lib().jgpm_serve_submit(self._h, graph._h, pattern_id(pattern), rb, re, ctypes.byref(t)),
+               "serve_submit")
+        return t.value
+
+    def poll(self, ticket):
+        c, us = ctypes.c_uint64(), ctypes.c_double()
+        st = lib().jgpm_serve_poll(self._h, ticket, ctypes.byref(c), ctypes.byref(us))
+        if st == -9:
+            return None
+        _check(st, "serve_poll")
+        return int(c.value), us.value
+
+    def wait(self, ticket):
+        c, us = ctypes.c_uint64(), ctypes.c_double()
+        _check(lib().jgpm_serve_wait(self._h, ticket, ctypes.byref(c), ctypes.byref(us)), "serve_wait")
+        return int(c.value), us.value
+
+    def stop(self):
+        if self._h:
+            lib().jgpm_serve_stop(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()

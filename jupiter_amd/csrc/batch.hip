@@ -22,19 +22,6 @@
 
 namespace jk {
 
-// Query descriptor as the persistent kernel sees it.
-struct BQ {
-  const int64_t* rp;
-  const int32_t* ci;
-  const int32_t* src;
-  const int32_t* tri;
-  const int64_t* nsum;
-  int64_t base;    // first root (clique) or first edge id (P1..P12)
-  int64_t ntasks;  // roots or edges
-  int32_t kind;    // 3..7: clique size; 100 + P: symmetric pattern P
-  int32_t chunk;   // roots / edges per chunk
-};
-
 struct BatchArgs {
   const BQ* q;
   const int64_t* cstart;  // chunk prefix, nq + 1 entries
@@ -225,12 +212,18 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
       b.base = g->sym_rp_h[rb];
       b.ntasks = g->sym_rp_h[re] - b.base;
       b.chunk = (p == 2 || p == 3) ? 64 : 8;
-      max_sym_deg = std::max(max_sym_deg, g->sym.max_deg);
+      max_sym_deg = std::max(max_sym_deg, g->sym_mmin);
     }
   }
   cs.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) cs[i + 1] = cs[i] + (hq[i].ntasks + hq[i].chunk - 1) / hq[i].chunk;
-  const int grid = std::max(1, c->tune.blocks_per_cu) * c->num_cu;
+  if (c->batch_blocks_per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, jk::batch_kernel, 64 * jk::BWPB, 0) != hipSuccess || nb < 1)
+      nb = 2;
+    c->batch_blocks_per_cu = nb;
+  }
+  const int grid = std::min(std::max(1, c->tune.blocks_per_cu), c->batch_blocks_per_cu) * c->num_cu;
   const size_t need = (size_t)grid * jk::BWPB * jk::symd::NLIST * (size_t)max_sym_deg;
   if (c->sscr_ints < need) {
     if (hipStreamSynchronize(st) != hipSuccess) return JGPM_E_HIP;
@@ -269,6 +262,13 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
     if (rc) return rc;
   }
   return JGPM_OK;
+}
+
+bool batch_supported(int pattern) { return batch_kind(pattern) != 0; }
+
+int enqueue_batch_q(jgpm_ctx_s* c, const jgpm_query* q, int n, hipStream_t st) {
+  // the staging lives on the context until the next batched launch
+  return enqueue_batch(c, q, n, c->bq_host, c->bcs_host, st);
 }
 
 }  // namespace jgpm

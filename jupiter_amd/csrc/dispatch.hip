@@ -52,7 +52,7 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   if (rc) return rc;
   const int wpb = jk::sym_waves_per_block();
   const int grid = grid_for(c, 2);
-  const size_t stride = (size_t)std::max(1, g->sym.max_deg);
+  const size_t stride = (size_t)std::max(1, g->sym_mmin);
   const size_t need = (size_t)grid * wpb * jk::sym_lists_per_wave() * stride;
   if (c->sscr_ints < need) {
     if (c->d_sscr) {

@@ -146,11 +146,28 @@ static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_
     g->has_sym = true;
     g->bytes += 8 * (int64_t)(sym->nv + 1) + 4 * sym->nnz;
     g->sym_rp_h = sym->rowptr;
+    // longest intersection list of any edge: max over edges of min(d(u), d(v))
+    std::atomic<int32_t> best{1};
+    parallel_for(sym->nv, 4096, [&](int64_t b, int64_t e) {
+      int32_t m = 1;
+      for (int64_t u = b; u < e; ++u) {
+        const int32_t du = (int32_t)(sym->rowptr[u + 1] - sym->rowptr[u]);
+        for (int64_t k = sym->rowptr[u]; k < sym->rowptr[u + 1]; ++k) {
+          const int32_t v = sym->col[k];
+          m = std::max(m, std::min(du, (int32_t)(sym->rowptr[v + 1] - sym->rowptr[v])));
+        }
+      }
+      int32_t cur = best.load();
+      while (m > cur && !best.compare_exchange_weak(cur, m)) {
+      }
+    });
+    g->sym_mmin = best.load();
   }
   st = upload_csr(g->dag, dag->rowptr.data(), dag->col.data(), dag->nv, dag->nnz, dag->max_deg);
   if (st) { jgpm_graph_free(g); return st; }
   g->has_dag = true;
   g->bytes += 8 * (int64_t)(dag->nv + 1) + 4 * dag->nnz;
+  g->dag_rp_h = dag->rowptr;
   st = build_tiers(g, dag->rowptr.data(), dag->nv);
   if (st) { jgpm_graph_free(g); return st; }
   *out = g;

@@ -39,6 +39,9 @@ struct jgpm_ctx_s {
   uint64_t* d_bcnt = nullptr;
   unsigned* d_bq32 = nullptr;  // queue counters of the follow-up tier launches
   int bcap = 0;
+  int batch_blocks_per_cu = 0;  // resident blocks of the batched kernel
+  std::vector<jk::BQ> bq_host; // staging of the last batched launch
+  std::vector<int64_t> bcs_host;
   uint64_t* d_scr = nullptr;  // row scratch of the global-row tier
   size_t scr_words = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -59,6 +62,8 @@ struct jgpm_graph_s {
   int32_t* tier_d[jk::NUM_TIERS] = {nullptr};
   std::vector<int32_t> huge_h;                 // roots beyond the largest tier
   std::vector<int64_t> sym_rp_h;               // host copy of the symmetric row pointers
+  std::vector<int64_t> dag_rp_h;               // host copy of the DAG row pointers
+  int32_t sym_mmin = 1;                        // max over edges of min(d(u), d(v)): longest S = N(u) ∩ N(v)
   int32_t* d_src = nullptr;                    // COO source per symmetric edge (lazy)
   int32_t* d_tri = nullptr;                    // triangles per symmetric edge (lazy)
   int64_t* d_nsum = nullptr;                   // neighbour-degree sums (lazy)
@@ -73,6 +78,11 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
                    bool split_slots = false, int first_tier = 0, uint64_t* out_ptr = nullptr,
                    unsigned* qctr_base = nullptr);
 int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st);
+// Batched persistent launch over queries the kernel supports (TC, P1..P16);
+// counts land in c->d_bcnt[0..n).  hq / cs are host staging that must stay
+// alive until the stream is synchronised.
+int enqueue_batch_q(jgpm_ctx_s* c, const jgpm_query* q, int n, hipStream_t st);
+bool batch_supported(int pattern);
 bool pattern_uses_tri(int p);
 // profile hooks around one kernel launch on the context stream
 void prof_begin(jgpm_ctx_s* c, hipStream_t st, const char* name);

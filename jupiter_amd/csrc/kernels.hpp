@@ -30,6 +30,19 @@ struct CliqueArgs {
   uint64_t* out;         // uint64 accumulator
 };
 
+// Query descriptor as the persistent kernel sees it.
+struct BQ {
+  const int64_t* rp;
+  const int32_t* ci;
+  const int32_t* src;
+  const int32_t* tri;
+  const int64_t* nsum;
+  int64_t base;    // first root (clique) or first edge id (P1..P12)
+  int64_t ntasks;  // roots or edges
+  int32_t kind;    // 3..7: clique size; 100 + P: symmetric pattern P
+  int32_t chunk;   // roots / edges per chunk
+};
+
 hipError_t launch_clique(int k, bool bytes, int tier, const CliqueArgs& a, int grid, hipStream_t st);
 
 // Symmetric-graph patterns P1..P12: one wave per directed edge of [e0, e0+ne).

@@ -1,0 +1,259 @@
+// serve.hip — continuous-batching request loop over device-resident graphs.
+// No reference counterpart (the reference runs one query per process).
+//
+// A scheduler thread owns the context's stream while the server runs:
+//  * admission: a submitted request is admitted when the number of requests
+//    on the device is below max_inflight and the HBM its intermediates need
+//    (per-graph auxiliary arrays it would create plus its share of the list
+//    scratch) fits the budget; a request that could never fit is rejected
+//    at submit with JGPM_E_BUDGET;
+//  * slicing: every admitted request is cut into root slices of bounded work
+//    (edges of the slice times a per-pattern cost weight), so expensive
+//    queries cannot hold the device for long;
+//  * iteration: each loop takes the next slice of every admitted request and
+//    runs them all in one persistent batched launch; requests that arrived
+//    meanwhile are admitted before the next launch, so their work overlaps
+//    with the tails of earlier ones on the device;
+//  * completion: a request finishes when its last slice is counted; its
+//    latency is submit -> result visible on the host.
+// Counts are sums over disjoint root slices, so a result is bit-identical to
+// the single-query count of the same (graph, pattern, root range).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "jgpm_device.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Req {
+  uint64_t ticket;
+  jgpm_graph_t g;
+  int pattern;
+  int64_t rb, re, cur;
+  uint64_t count = 0;
+  int64_t footprint = 0;
+  clk::time_point t_submit;
+};
+
+struct Done {
+  uint64_t count;
+  double latency_us;
+  int status;
+};
+
+// relative cost per edge of a slice, used only to bound slice work
+double pattern_weight(int p) {
+  switch (p) {
+    case JGPM_TC: case JGPM_P2: return 1;
+    case JGPM_P13: case JGPM_P3: case JGPM_P5: case JGPM_P7: case JGPM_P9: return 4;
+    case JGPM_P12: case JGPM_P11: return 16;
+    case JGPM_P14: case JGPM_P1: case JGPM_P4: case JGPM_P8: return 32;
+    case JGPM_P6: case JGPM_P10: case JGPM_P15: return 128;
+    case JGPM_P16: return 512;
+  }
+  return 8;
+}
+
+}  // namespace
+
+struct jgpm_serve_s {
+  jgpm_ctx_s* ctx = nullptr;
+  jgpm_serve_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req> pending;
+  std::vector<Req> active;
+  std::unordered_map<uint64_t, Done> done;
+  uint64_t next_ticket = 1;
+  int64_t inflight_bytes = 0;
+  bool stop = false;
+  int fatal = JGPM_OK;
+  std::thread th;
+  int64_t slice_edges = 1 << 22;
+};
+
+static bool is_dag_pattern(int p) { return jgpm::clique_k(p) != 0 && p != JGPM_I_K4; }
+
+// HBM the request's intermediates need beyond the resident graph.
+static int64_t footprint(jgpm_graph_s* g, int p, const jgpm_ctx_s* c, int grid) {
+  int64_t b = 64;  // count slot and queue counters
+  if (!is_dag_pattern(p)) {
+    if (!g->d_src) b += 4 * g->sym.nnz;
+    if (!g->d_tri && jgpm::pattern_uses_tri(p)) b += 4 * g->sym.nnz;
+    if (!g->d_nsum && p == JGPM_P3) b += 8 * (int64_t)g->sym.nv;
+    b += (int64_t)grid * 4 * 5 * 4 * std::max(1, g->sym_mmin);  // list scratch of the batch kernel
+  } else if (g->dag.max_deg > jk::tier_max_deg(jk::TIER_LDS16)) {
+    b += (int64_t)4 * c->num_cu * (int64_t)jk::tier_scratch_words(jk::TIER_GLB32) * 8;
+  }
+  (void)c;
+  return b;
+}
+
+static void serve_loop(jgpm_serve_s* s) {
+  jgpm_ctx_s* c = s->ctx;
+  std::vector<jgpm_query> qs;
+  std::vector<size_t> owner;
+  std::vector<uint64_t> h;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->active.empty(); });
+      if (s->stop && s->active.empty() && s->pending.empty()) return;
+      // admission in arrival order
+      while (!s->pending.empty() && (int)s->active.size() < s->cfg.max_inflight) {
+        Req& r = s->pending.front();
+        if (!s->active.empty() && s->inflight_bytes + r.footprint > s->cfg.hbm_budget_bytes) break;
+        s->inflight_bytes += r.footprint;
+        s->active.push_back(r);
+        s->pending.pop_front();
+      }
+    }
+    // next slice of every active request (the active list is only touched here)
+    qs.clear();
+    owner.clear();
+    for (size_t i = 0; i < s->active.size(); ++i) {
+      Req& r = s->active[i];
+      if (r.cur >= r.re) continue;
+      jgpm_graph_s* g = r.g;
+      const std::vector<int64_t>& rp = is_dag_pattern(r.pattern) ? g->dag_rp_h : g->sym_rp_h;
+      const int64_t budget = std::max<int64_t>(1, (int64_t)(s->slice_edges / pattern_weight(r.pattern)));
+      int64_t end = std::upper_bound(rp.begin() + r.cur, rp.begin() + r.re + 1, rp[r.cur] + budget) - rp.begin() - 1;
+      end = std::max(end, r.cur + 1);
+      if (s->cfg.slice_roots > 0) end = std::min(end, r.cur + s->cfg.slice_roots);
+      end = std::min(end, r.re);
+      qs.push_back(jgpm_query{g, r.pattern, 0, r.cur, end});
+      owner.push_back(i);
+      r.cur = end;
+    }
+    int st = JGPM_OK;
+    if (!qs.empty()) {
+      std::lock_guard<std::mutex> lk(c->mu);
+      st = (hipSetDevice(c->dev) == hipSuccess) ? JGPM_OK : JGPM_E_HIP;
+      if (!st) st = jgpm::ctx_reset_counters(c, c->st);
+      if (!st) {
+        c->nprof = 0;
+        st = jgpm::enqueue_batch_q(c, qs.data(), (int)qs.size(), c->st);
+      }
+      h.assign(qs.size(), 0);
+      if (!st && hipMemcpyAsync(h.data(), c->d_bcnt, sizeof(uint64_t) * qs.size(), hipMemcpyDeviceToHost, c->st) !=
+                     hipSuccess)
+        st = JGPM_E_HIP;
+      if (hipStreamSynchronize(c->st) != hipSuccess && !st) st = JGPM_E_HIP;
+    }
+    const auto now = clk::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t k = 0; k < qs.size(); ++k) s->active[owner[k]].count += h[k];
+    // finished requests (or all, after an error) leave the device
+    std::vector<Req> keep;
+    for (Req& r : s->active) {
+      if (st || r.cur >= r.re) {
+        double us = std::chrono::duration<double, std::micro>(now - r.t_submit).count();
+        s->done[r.ticket] = Done{r.count, us, st};
+        s->inflight_bytes -= r.footprint;
+      } else {
+        keep.push_back(r);
+      }
+    }
+    s->active.swap(keep);
+    if (st) s->fatal = st;
+    s->cv_done.notify_all();
+  }
+}
+
+extern "C" int jgpm_serve_start(jgpm_ctx_t ctx, const jgpm_serve_config* cfg, jgpm_serve_t* out) {
+  if (!ctx || !out) return JGPM_E_INVALID;
+  auto* s = new (std::nothrow) jgpm_serve_s();
+  if (!s) return JGPM_E_NOMEM;
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = ctx->hbm / 2;
+  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 256;
+  s->th = std::thread(serve_loop, s);
+  *out = s;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, int64_t rb, int64_t re,
+                                 uint64_t* ticket) {
+  if (!s || !g || !ticket || g->ctx != s->ctx) return JGPM_E_INVALID;
+  if (pattern < 0 || pattern >= JGPM_NUM_PATTERNS) return JGPM_E_PATTERN;
+  if (!jgpm::batch_supported(pattern)) return JGPM_E_PATTERN;  // served ids: TC, P1..P16
+  const int64_t nv = g->dag.nv;
+  if (re < 0) re = nv;
+  if (rb < 0 || rb > re || re > nv) return JGPM_E_INVALID;
+  if (!is_dag_pattern(pattern) && !g->has_sym) return JGPM_E_GRAPHKIND;
+  if (is_dag_pattern(pattern) && !g->huge_h.empty()) {
+    auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
+    if (lo != g->huge_h.end() && *lo < re) return JGPM_E_BUDGET;
+  }
+  const int grid = std::max(1, std::min(s->ctx->tune.blocks_per_cu, s->ctx->batch_blocks_per_cu > 0
+                                                                      ? s->ctx->batch_blocks_per_cu : 3)) *
+                   s->ctx->num_cu;
+  Req r;
+  r.g = g;
+  r.pattern = pattern;
+  r.rb = rb;
+  r.re = re;
+  r.cur = rb;
+  r.footprint = footprint(g, pattern, s->ctx, grid);
+  r.t_submit = clk::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return JGPM_E_STATE;
+  if (s->fatal) return s->fatal;
+  if (r.footprint > s->cfg.hbm_budget_bytes) return JGPM_E_BUDGET;
+  r.ticket = s->next_ticket++;
+  *ticket = r.ticket;
+  if (rb == re) {  // empty range: done at once
+    s->done[r.ticket] = Done{0, 0.0, JGPM_OK};
+    s->cv_done.notify_all();
+    return JGPM_OK;
+  }
+  s->pending.push_back(r);
+  s->cv_work.notify_one();
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_serve_poll(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us) {
+  if (!s || !count) return JGPM_E_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->done.find(ticket);
+  if (it == s->done.end()) return ticket == 0 || ticket >= s->next_ticket ? JGPM_E_INVALID : JGPM_E_NOTREADY;
+  if (it->second.status) return it->second.status;
+  *count = it->second.count;
+  if (latency_us) *latency_us = it->second.latency_us;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_serve_wait(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us) {
+  if (!s || !count) return JGPM_E_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (ticket == 0 || ticket >= s->next_ticket) return JGPM_E_INVALID;
+  s->cv_done.wait(lk, [&] { return s->done.count(ticket) > 0; });
+  const Done& d = s->done[ticket];
+  if (d.status) return d.status;
+  *count = d.count;
+  if (latency_us) *latency_us = d.latency_us;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_serve_stop(jgpm_serve_t s) {
+  if (!s) return JGPM_E_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  delete s;
+  return JGPM_OK;
+}

@@ -1,0 +1,60 @@
+"""Serving loop: concurrently submitted queries of different patterns and
+graphs give the same uint64 count as alone (tolerance 0); admission by the
+HBM budget rejects what cannot fit."""
+import threading
+
+import numpy as np
+import pytest
+
+import jupiter_amd as J
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_concurrent_requests_equal_alone(ctx):
+    graphs = [ctx.upload(J.rmat(11, 16, seed=s)) for s in (1, 2)]
+    graphs.append(ctx.upload(J.barabasi_albert(4000, 6, seed=3)))
+    graphs.append(ctx.upload(J.erdos_renyi(3000, 16.0, seed=4)))
+    rng = np.random.default_rng(0)
+    pats = ["TC"] + [f"P{i}" for i in range(1, 17)]
+    reqs = []
+    for i in range(60):
+        G = graphs[int(rng.integers(0, len(graphs)))]
+        p = pats[int(rng.integers(0, len(pats)))]
+        nv = G.info()["nv"]
+        if i % 3 == 0:
+            a = int(rng.integers(0, nv))
+            b = int(rng.integers(a, nv + 1))
+            reqs.append((G, p, (a, b)))
+        else:
+            reqs.append((G, p, None))
+    alone = [G.count(p, r)[0] for G, p, r in reqs]
+    results = [None] * len(reqs)
+    with J.Server(ctx, slice_roots=512) as srv:
+        def worker(ids):
+            for k in ids:
+                t = srv.submit(reqs[k][0], reqs[k][1], reqs[k][2])
+                results[k] = srv.wait(t)
+        ths = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 4)),)) for j in range(4)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+    assert [r[0] for r in results] == alone
+    assert all(r[1] > 0 for r in results)
+    for G in graphs:
+        G.free()
+
+
+def test_budget_and_pattern_rejection(ctx):
+    G = ctx.upload(J.rmat(10, 16, seed=1))
+    with J.Server(ctx, hbm_budget_bytes=1 << 16) as srv:
+        with pytest.raises(J.JgpmError) as e:
+            srv.submit(G, "P3")
+        assert e.value.status == -6
+        with pytest.raises(J.JgpmError) as e:
+            srv.submit(G, "I_C4")
+        assert e.value.status == -2
+        t = srv.submit(G, "TC", (5, 5))
+        assert srv.wait(t) == (0, 0.0)
+    G.free()
