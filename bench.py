@@ -67,7 +67,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scale", type=int, default=SCALE)
+    ap.add_argument("--profile", action="store_true",
+                    help="only the counting kernels of one step (for rocprofv3 counter runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one plain 1 GiB device-to-device copy kernel (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the count shards by root "
               "range; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -81,6 +93,13 @@ def main():
         G = ctx.upload(g)
         info = G.info()
         log(f"device {ctx.info()} graph {info}")
+        if args.profile:
+            for p in QUERIES:
+                c, ms = G.count(p, (0, q))
+                print(json.dumps({"pattern": p, "count": c, "kernel_ms": ms,
+                                  "kernels": ctx.last_profile()}), flush=True)
+            G.free()
+            return
         for w in range(args.warmup):
             for p in QUERIES:
                 c, ms = G.count(p, (0, q))
