@@ -60,6 +60,141 @@ def cpu_baseline(dag, roots_end, frac_den=1024, seed=7):
                       f"{t_total:.1f}s measured, scaled x{frac_den}"}
 
 
+def splitmix(seed, i):
+    x = (seed * 0x9E3779B97F4A7C15 + i * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x ^= x >> 31
+    return (x * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+
+
+def workload_batch(args):
+    """Many independent graphs, each paired with one pattern from
+    {TC, P13, P1, P2}: one persistent batched launch vs one plain launch per
+    query.  Full size: --batch-graphs 2048 --batch-max-n 262144."""
+    rng = np.random.default_rng(SEED)
+    specs = []
+    t0 = time.time()
+    csrs = []
+    for i in range(args.batch_graphs):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(args.batch_max_n))))
+        sd = splitmix(SEED, i)
+        if i % 2 == 0:
+            md = float(rng.uniform(8, 64))
+            csrs.append(J.erdos_renyi(n, md, seed=sd))
+            kind = f"ER(n={n},deg={md:.1f})"
+        else:
+            m = int(rng.integers(4, 17))
+            csrs.append(J.barabasi_albert(n, m, seed=sd))
+            kind = f"BA(n={n},m={m})"
+        specs.append((kind, ["TC", "P13", "P1", "P2"][int(rng.integers(0, 4))]))
+    edges = sum(c.nnz // 2 for c in csrs)
+    log(f"generated {len(csrs)} graphs, {edges} edges ({time.time() - t0:.1f}s)")
+    with J.Context(0) as ctx:
+        Gs = [ctx.upload(c) for c in csrs]
+        queries = [(G, p, 0, None) for G, (_, p) in zip(Gs, specs)]
+        for _ in range(args.warmup):
+            ctx.count_batch(queries)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch_counts, batch_ms = ctx.count_batch(queries)
+        t_batch = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        plain_counts = [G.count(p)[0] for G, p, _, _ in queries]
+        t_plain = time.perf_counter() - t0
+        if batch_counts != plain_counts:
+            bad = [i for i, (x, y) in enumerate(zip(batch_counts, plain_counts)) if x != y]
+            raise AssertionError(f"batched counts differ from plain counts at {bad[:10]}")
+        for G in Gs:
+            G.free()
+    nq = len(queries)
+    return {"metric": METRIC, "value": round(nq / t_batch, 3), "unit": "queries/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": f"batched: {nq} graphs (half ER mean degree U[8,64], half BA m U{{4..16}}, "
+                                   f"n log-uniform [1k,{args.batch_max_n}]), pattern uniform from "
+                                   "{TC,P13,P1,P2}, one persistent launch per step",
+                       "edges": edges, "plain_queries_per_s": round(nq / t_plain, 3),
+                       "speedup_vs_plain": round(t_plain / t_batch, 3), "counts_equal_plain": True}}
+
+
+SERVE_CAP = {"TC": 99, "P2": 99, "P13": 99, "P3": 99, "P5": 99, "P7": 99, "P9": 99, "P14": 99,
+             "P1": 14, "P11": 14, "P12": 14, "P4": 12, "P8": 12, "P15": 12, "P6": 11, "P10": 10, "P16": 10}
+
+
+def workload_serve(args):
+    """Poisson request stream over resident R-MAT graphs, each request a
+    uniform pattern id (TC, P1..P16) on a uniform graph (within the
+    pattern's scale cap), served with continuous batching vs sequential plain
+    launches (FIFO, one query per launch).  Counts must be identical."""
+    import threading
+    rng = np.random.default_rng(SEED)
+    scales = [int(rng.integers(args.serve_min_scale, args.serve_max_scale + 1)) for _ in range(32)]
+    t0 = time.time()
+    csrs = [J.rmat(sc, EF, 0.57, 0.19, 0.19, seed=splitmix(SEED, i)) for i, sc in enumerate(scales)]
+    log(f"generated 32 R-MAT graphs scales {sorted(scales)} ({time.time() - t0:.1f}s)")
+    pats = ["TC"] + [f"P{i}" for i in range(1, 17)]
+    out = {"rates": {}}
+    with J.Context(0) as ctx:
+        Gs = [ctx.upload(c) for c in csrs]
+        for rate in args.rates:
+            reqs = []
+            for _ in range(args.requests):
+                p = pats[int(rng.integers(0, len(pats)))]
+                ok = [i for i, sc in enumerate(scales) if sc <= SERVE_CAP[p]]
+                reqs.append((int(rng.choice(ok)), p))
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+            # served
+            res = [None] * len(reqs)
+            with J.Server(ctx) as srv:
+                tickets = [None] * len(reqs)
+                start = time.perf_counter()
+
+                def submitter():
+                    for k, (gi, p) in enumerate(reqs):
+                        dt = arrivals[k] - (time.perf_counter() - start)
+                        if dt > 0:
+                            time.sleep(dt)
+                        tickets[k] = srv.submit(Gs[gi], p)
+                th = threading.Thread(target=submitter)
+                th.start()
+                th.join()
+                for k in range(len(reqs)):
+                    res[k] = srv.wait(tickets[k])
+                span = time.perf_counter() - start
+            lat = np.array([r[1] for r in res]) / 1e3
+            # sequential plain launches, FIFO from the same arrival times
+            plain = []
+            plat = []
+            start = time.perf_counter()
+            for k, (gi, p) in enumerate(reqs):
+                dt = arrivals[k] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                c, _ = Gs[gi].count(p)
+                plain.append(c)
+                plat.append((time.perf_counter() - start - arrivals[k]) * 1e3)
+            pspan = time.perf_counter() - start
+            if [r[0] for r in res] != plain:
+                raise AssertionError(f"served counts differ from plain counts at rate {rate}")
+            out["rates"][str(rate)] = {
+                "served_qps": round(len(reqs) / span, 2), "served_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "served_p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "plain_qps": round(len(reqs) / pspan, 2),
+                "plain_p50_ms": round(float(np.percentile(plat, 50)), 3),
+                "plain_p99_ms": round(float(np.percentile(plat, 99)), 3)}
+            log(f"rate {rate}: {out['rates'][str(rate)]}")
+        for G in Gs:
+            G.free()
+    best = max(out["rates"].values(), key=lambda r: r["served_qps"])
+    return {"metric": METRIC, "value": best["served_qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(args.rates),
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32", "data": "synthetic",
+            "config": {"workload": f"serving: 32 resident R-MAT graphs scales {args.serve_min_scale}-"
+                                   f"{args.serve_max_scale} (ef 16), Poisson arrivals, uniform pattern id "
+                                   "(TC,P1..P16) with per-pattern scale caps, continuous batching vs "
+                                   "sequential plain launches", "scale_caps": SERVE_CAP,
+                       "requests_per_rate": args.requests, **out}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +204,13 @@ def main():
     ap.add_argument("--scale", type=int, default=SCALE)
     ap.add_argument("--profile", action="store_true",
                     help="only the counting kernels of one step (for rocprofv3 counter runs)")
+    ap.add_argument("--workload", choices=["cliques", "batch", "serve"], default="cliques")
+    ap.add_argument("--batch-graphs", type=int, default=256)
+    ap.add_argument("--batch-max-n", type=int, default=65536)
+    ap.add_argument("--serve-min-scale", type=int, default=10)
+    ap.add_argument("--serve-max-scale", type=int, default=16)
+    ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 2000])
+    ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one plain 1 GiB device-to-device copy kernel (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -85,6 +227,12 @@ def main():
               "range; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        print(json.dumps(workload_batch(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(workload_serve(args)), flush=True)
+        return
     t0 = time.time()
     g = J.rmat(args.scale, EF, 0.57, 0.19, 0.19, seed=SEED)
     log(f"R-MAT scale {args.scale}: nv={g.nv} sym_nnz={g.nnz} ({time.time() - t0:.1f}s)")

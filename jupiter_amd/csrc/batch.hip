@@ -36,15 +36,15 @@ struct BatchArgs {
 constexpr int BWPB = 4;  // independent waves per block
 
 template <int K>
-__device__ __forceinline__ void batch_root(const int64_t* rp, const int32_t* ci, int64_t b0, int d, int ratio,
-                                           unsigned char* ar, int32_t* wtot, uint64_t& acc) {
+__device__ __forceinline__ void batch_root(const int64_t* rp, const int32_t* ci, const int32_t* ord, int64_t b0,
+                                           int d, int ratio, unsigned char* ar, int32_t* wtot, uint64_t& acc) {
   if (d <= 64) {
     using LY = Lay<1, 1, K, false, true>;
-    process_root<1, 1, K, false, true>(rp, ci, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
+    process_root<1, 1, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
                                        wtot, acc);
   } else {
     using LY = Lay<1, 4, K, false, true>;
-    process_root<1, 4, K, false, true>(rp, ci, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
+    process_root<1, 4, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
                                        wtot, acc);
   }
 }
@@ -103,11 +103,11 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
         const int d = (int)(Q.rp[v0 + 1] - b0);
         if (d < Q.kind - 1 || d > 256) continue;
         switch (Q.kind) {
-          case 3: batch_root<3>(Q.rp, Q.ci, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
-          case 4: batch_root<4>(Q.rp, Q.ci, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
-          case 5: batch_root<5>(Q.rp, Q.ci, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
-          case 6: batch_root<6>(Q.rp, Q.ci, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
-          case 7: batch_root<7>(Q.rp, Q.ci, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 3: batch_root<3>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 4: batch_root<4>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 5: batch_root<5>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 6: batch_root<6>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 7: batch_root<7>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
         }
       } else {
         const int64_t e = Q.base + u;
@@ -197,6 +197,7 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
     if (kind < 100) {
       b.rp = g->dag.rp;
       b.ci = g->dag.ci;
+      b.ord = g->d_ord;
       b.base = rb;
       b.ntasks = re - rb;
       b.chunk = 16;

@@ -268,8 +268,10 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
       if constexpr (BYTES) s += wt[x];
       if constexpr (DEPTH > 1) {
         uint64_t nx[WM];
+        // rows hold successors only (positions after x): skip the words before
+        const int ws = (b + 1) >> 6;
 #pragma unroll
-        for (int w = 0; w < WM; ++w) nx[w] = (w < nw) ? (c[w] & R(x, wlo + w)) : 0ull;
+        for (int w = 0; w < WM; ++w) nx[w] = (w >= ws && w < nw) ? (c[w] & R(x, wlo + w)) : 0ull;
         s += recR<DEPTH - 1, WM, BYTES>(nx, wlo, nw, R, wt);
       }
     }
@@ -296,8 +298,10 @@ struct Lay {
   // build view
   static constexpr size_t B_OFFS = 0;
   static constexpr size_t B_UB = al(B_OFFS + 4 * (DM + 1));
-  static constexpr size_t B_HDU = al(B_UB + 8 * DM);
-  static constexpr size_t B_END = al(B_HDU + 4 * DM);
+  static constexpr size_t B_HFL = al(B_UB + 8 * DM);                    // u32 [DM/32] long-list flags
+  static constexpr size_t B_KEY = al(B_HFL + 4 * ((DM + 31) / 32));      // int32 [DM] order keys
+  static constexpr size_t B_RNK = al(B_KEY + (MULTI ? 4 * DM : 0));      // int16 [DM] rank in DAG order
+  static constexpr size_t B_END = al(B_RNK + (MULTI ? 2 * DM : 0));
   // count view
   static constexpr size_t C_J = 0;                                          // int16 [G][DM]
   static constexpr size_t C_JB = al(C_J + (MULTI ? 2 * G * DM : 0));        // (unused)
@@ -316,8 +320,9 @@ struct Lay {
 // with BYTES, its work-model bytes) to `acc`.  `ar` is the group's LDS
 // arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes.
 template <int G, int WMAX, int K, bool BYTES, bool LROWS>
-__device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, int64_t b0, int d,
-                             int ratio, unsigned char* ar, uint64_t* rowp, int32_t* s_wtot, uint64_t& acc) {
+__device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                             const int32_t* __restrict__ ord, int64_t b0, int d, int ratio, unsigned char* ar,
+                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc) {
   using LY = Lay<G, WMAX, K, BYTES, LROWS>;
   constexpr int DM = LY::DM;
   constexpr bool NEED_ROWS = LY::ROWS;
@@ -329,13 +334,19 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   unsigned char* U = ar + LY::U_OFF;
   int32_t* off = reinterpret_cast<int32_t*>(U + LY::B_OFFS);
   int64_t* ub = reinterpret_cast<int64_t*>(U + LY::B_UB);
-  int32_t* hdu = reinterpret_cast<int32_t*>(U + LY::B_HDU);
+  uint32_t* hfl = reinterpret_cast<uint32_t*>(U + LY::B_HFL);
+  int32_t* key = reinterpret_cast<int32_t*>(U + LY::B_KEY);
+  int16_t* rnk = reinterpret_cast<int16_t*>(U + LY::B_RNK);
+  // Multi-word roots number their members in DAG order (rank of ord[]), so
+  // every row only has bits above its own position.
+  constexpr bool RANK = LY::MULTI;
   const int W = (d + 63) >> 6;
   if constexpr (BYTES) {
     if (r == 0) acc += 16 + 4 * (uint64_t)d;
   }
   // ---- stage root list and segment descriptors
   for (int t = r; t < d; t += Grp<G>::size) L[t] = ci[b0 + t];
+  for (int t = r; t < (d + 31) / 32; t += Grp<G>::size) hfl[t] = 0u;
   Grp<G>::sync();
   for (int t = r; t < d; t += Grp<G>::size) {
     int32_t u = L[t];
@@ -344,10 +355,24 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     bool heavy = du > 64 && du > ratio * d;
     ub[t] = s;
     off[t] = heavy ? 0 : du;
-    hdu[t] = heavy ? du : 0;
-    if constexpr (BYTES) {
-      wL[t] = 16u + 4u * (uint32_t)du;
-      acc += 16 + 4 * (uint64_t)du;  // f(v1) of the edge (v0, L[t])
+    if (heavy) atomicOr(&hfl[t >> 5], 1u << (t & 31));
+    if constexpr (RANK) key[t] = ord[u];
+    if constexpr (BYTES) acc += 16 + 4 * (uint64_t)du;  // f(v1) of the edge (v0, L[t])
+  }
+  if constexpr (RANK) {
+    Grp<G>::sync();
+    for (int t = r; t < d; t += Grp<G>::size) {
+      const int32_t kt = key[t];
+      int c = 0;
+      for (int s2 = 0; s2 < d; ++s2) c += key[s2] < kt;
+      rnk[t] = (int16_t)c;
+    }
+  }
+  Grp<G>::sync();
+  if constexpr (BYTES) {
+    for (int t = r; t < d; t += Grp<G>::size) {
+      const int pos = RANK ? (int)rnk[t] : t;
+      wL[pos] = 16u + 4u * (uint32_t)(rp[L[t] + 1] - ub[t]);
     }
   }
   if constexpr (NEED_ROWS) {
@@ -367,23 +392,32 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       if constexpr (K == 3) {
         ++acc;
       } else {
-        atomicOr((unsigned long long*)&rowp[i * W + (p >> 6)], 1ull << (p & 63));
+        const int ri = RANK ? (int)rnk[i] : i;
+        const int rpb = RANK ? (int)rnk[p] : p;
+        atomicOr((unsigned long long*)&rowp[ri * W + (rpb >> 6)], 1ull << (rpb & 63));
       }
     }
   }
   // ---- long lists: binary search every element of L in them
   for (int i = 0; i < d; ++i) {
-    const int du = hdu[i];
-    if (du == 0) continue;
+    if (!((hfl[i >> 5] >> (i & 31)) & 1u)) continue;
+    const int du = (int)(rp[L[i] + 1] - ub[i]);
     const int32_t* lst = ci + ub[i];
+    const int ri = RANK ? (int)rnk[i] : i;
     for (int t0 = wv * 64; t0 < d; t0 += Grp<G>::size) {
       int t = t0 + ln;
       bool f = t < d && bsearch_global(lst, du, L[t]);
-      uint64_t m = __ballot(f);
       if constexpr (K == 3) {
+        uint64_t m = __ballot(f);
         if (ln == 0) acc += (uint64_t)__popcll(m);
+      } else if constexpr (RANK) {
+        if (f) {
+          const int rt = rnk[t];
+          atomicOr((unsigned long long*)&rowp[ri * W + (rt >> 6)], 1ull << (rt & 63));
+        }
       } else {
-        if (ln == 0 && m) atomicOr((unsigned long long*)&rowp[i * W + (t0 >> 6)], m);
+        uint64_t m = __ballot(f);
+        if (ln == 0 && m) atomicOr((unsigned long long*)&rowp[ri * W + (t0 >> 6)], m);
       }
     }
   }
@@ -431,7 +465,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
             acc += wL[j];
           } else {
             uint64_t c = 0;
-            for (int w = wlo; w <= whi; ++w) c += __popcll(R(i, w) & R(j, w));
+            for (int w = max(wlo, (j + 1) >> 6); w <= whi; ++w) c += __popcll(R(i, w) & R(j, w));
             acc += c;
           }
         }
@@ -465,9 +499,10 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
           const int j = J[t];
           if constexpr (WMAX <= 16) {
             const int nw = whi - wlo + 1;
+            const int ws = ((j + 1) >> 6) - wlo;  // row j has no bits before its own position
             uint64_t c[WMAX];
 #pragma unroll
-            for (int w = 0; w < WMAX; ++w) c[w] = (w < nw) ? (R(i, wlo + w) & R(j, wlo + w)) : 0ull;
+            for (int w = 0; w < WMAX; ++w) c[w] = (w >= ws && w < nw) ? (R(i, wlo + w) & R(j, wlo + w)) : 0ull;
             if constexpr (BYTES) acc += wL[j] + recR<K - 4, WMAX, true>(c, wlo, nw, R, wL);
             else acc += recR<K - 3, WMAX, false>(c, wlo, nw, R, wL);
           } else {

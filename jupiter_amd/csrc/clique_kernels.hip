@@ -72,7 +72,7 @@ clique_kernel(CliqueArgs a) {
       const int64_t b0 = rp[v0];
       const int d = (int)(rp[v0 + 1] - b0);
       if (d < a.dmin || d > a.dmax || d > DM) continue;
-      process_root<G, WMAX, K, BYTES, LROWS>(rp, a.ci, b0, d, a.ratio, ar, rowp, s_wtot, acc);
+      process_root<G, WMAX, K, BYTES, LROWS>(rp, a.ci, a.ord, b0, d, a.ratio, ar, rowp, s_wtot, acc);
     }
   }
   block_add<NT>(acc, a.out);

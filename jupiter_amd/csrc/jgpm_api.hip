@@ -170,6 +170,40 @@ static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_
   g->dag_rp_h = dag->rowptr;
   st = build_tiers(g, dag->rowptr.data(), dag->nv);
   if (st) { jgpm_graph_free(g); return st; }
+  {
+    // a topological order of the DAG: (degree, id) when the symmetric degrees
+    // are known (the orientation rule), else Kahn's algorithm on the DAG
+    std::vector<int32_t> ord((size_t)dag->nv);
+    if (sym) {
+      int32_t md = sym->max_deg;
+      std::vector<int64_t> pos((size_t)md + 2, 0);
+      for (int32_t v = 0; v < sym->nv; ++v) pos[sym->rowptr[v + 1] - sym->rowptr[v] + 1]++;
+      for (int32_t k = 0; k <= md; ++k) pos[k + 1] += pos[k];
+      for (int32_t v = 0; v < sym->nv; ++v) ord[v] = (int32_t)pos[sym->rowptr[v + 1] - sym->rowptr[v]]++;
+    } else {
+      std::vector<int32_t> indeg((size_t)dag->nv, 0), q;
+      q.reserve((size_t)dag->nv);
+      for (int64_t k = 0; k < dag->nnz; ++k) indeg[dag->col[k]]++;
+      for (int32_t v = 0; v < dag->nv; ++v)
+        if (!indeg[v]) q.push_back(v);
+      for (size_t h = 0; h < q.size(); ++h) {
+        int32_t u = q[h];
+        ord[u] = (int32_t)h;
+        for (int64_t k = dag->rowptr[u]; k < dag->rowptr[u + 1]; ++k)
+          if (--indeg[dag->col[k]] == 0) q.push_back(dag->col[k]);
+      }
+      if ((int64_t)q.size() != dag->nv) { jgpm_graph_free(g); return JGPM_E_INVALID; }  // not acyclic
+    }
+    if (hipMalloc(&g->d_ord, sizeof(int32_t) * std::max<int64_t>(dag->nv, 1)) != hipSuccess) {
+      jgpm_graph_free(g);
+      return JGPM_E_NOMEM;
+    }
+    if (hipMemcpy(g->d_ord, ord.data(), sizeof(int32_t) * dag->nv, hipMemcpyHostToDevice) != hipSuccess) {
+      jgpm_graph_free(g);
+      return JGPM_E_HIP;
+    }
+    g->bytes += 4 * (int64_t)dag->nv;
+  }
   *out = g;
   return JGPM_OK;
 }
@@ -321,6 +355,7 @@ extern "C" int jgpm_graph_free(jgpm_graph_t g) {
   if (g->d_src) (void)hipFree(g->d_src);
   if (g->d_tri) (void)hipFree(g->d_tri);
   if (g->d_nsum) (void)hipFree(g->d_nsum);
+  if (g->d_ord) (void)hipFree(g->d_ord);
   delete g;
   return JGPM_OK;
 }
@@ -374,6 +409,7 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
     jk::CliqueArgs a{};
     a.rp = g->dag.rp;
     a.ci = g->dag.ci;
+    a.ord = g->d_ord;
     a.ratio = std::max(1, c->tune.gallop_ratio);
     a.qctr = qctr_base ? qctr_base + tier : c->d_q + (out_slot * jk::NUM_TIERS + tier) % CTX_SLOTS;
     a.out = out_ptr ? out_ptr : c->d_out + out_slot + (split_slots ? tier : 0);

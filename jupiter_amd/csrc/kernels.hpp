@@ -19,6 +19,7 @@ enum {
 struct CliqueArgs {
   const int64_t* rp;     // DAG row pointers
   const int32_t* ci;     // DAG column indices
+  const int32_t* ord;    // position of every vertex in a topological order of the DAG
   const int32_t* ids;    // task -> root vertex (NULL: root = rb + task)
   int64_t rb;
   int64_t ntasks;
@@ -34,6 +35,7 @@ struct CliqueArgs {
 struct BQ {
   const int64_t* rp;
   const int32_t* ci;
+  const int32_t* ord;
   const int32_t* src;
   const int32_t* tri;
   const int64_t* nsum;

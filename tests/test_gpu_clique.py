@@ -124,3 +124,16 @@ def test_gallop_ratio_does_not_change_counts(ctx):
             assert G.count(p)[0] == ent["counts"][p]
     ctx.set_tuning(gallop_ratio=32)
     G.free()
+
+
+def test_dag_only_upload_uses_topological_order(ctx):
+    """A DAG uploaded as such gets its member order from Kahn's algorithm;
+    counts match the oracle on roots wide enough for multi-word rows."""
+    g = J.rmat(13, 16, seed=9)
+    d = g.orient()
+    assert d.degrees().max() > 64
+    G = ctx.upload(d, flags=J.DAG)
+    for p in ["TC", "P13", "P14"]:
+        _say(f"dag-only {p}")
+        assert G.count(p)[0] == O.count(d.rowptr, d.col, J.PATTERNS[p]), p
+    G.free()
