@@ -7,8 +7,8 @@
 // the persistent grid takes the next chunk from one device-wide atomic queue
 // (chunks of expensive queries first), processes it with the same device
 // routines as the single-query kernels, and adds the chunk's count to that
-// query's uint64 slot.  Clique roots with out-degree <= 256 run inside the
-// wave (one- or four-word bitmap rows in LDS); the few wider roots of a query
+// query's uint64 slot.  Clique roots with out-degree <= 128 run inside the
+// wave (one- or two-word bitmap rows in LDS); the few wider roots of a query
 // run afterwards in the block-per-root tier kernels, into the same slot.
 #include <hip/hip_runtime.h>
 
@@ -43,8 +43,8 @@ __device__ __forceinline__ void batch_root(const int64_t* rp, const int32_t* ci,
     process_root<1, 1, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
                                        wtot, acc);
   } else {
-    using LY = Lay<1, 4, K, false, true>;
-    process_root<1, 4, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
+    using LY = Lay<1, 2, K, false, true>;
+    process_root<1, 2, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
                                        wtot, acc);
   }
 }
@@ -68,8 +68,8 @@ struct MaxI {
 };
 
 __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
-  constexpr int CL = MaxI<MaxI<Lay<1, 4, 7, false, true>::SLOT, Lay<1, 4, 5, false, true>::SLOT>::v,
-                          MaxI<Lay<1, 4, 4, false, true>::SLOT, Lay<1, 1, 7, false, true>::SLOT>::v>::v;
+  constexpr int CL = MaxI<MaxI<Lay<1, 2, 7, false, true>::SLOT, Lay<1, 2, 5, false, true>::SLOT>::v,
+                          MaxI<Lay<1, 2, 4, false, true>::SLOT, Lay<1, 1, 7, false, true>::SLOT>::v>::v;
   constexpr int SY = (symd::NLIST * symd::CAP + symd::STAGE) * 4;
   constexpr int ARENA = ((CL > SY ? CL : SY) + 15) & ~15;
   __shared__ __attribute__((aligned(16))) unsigned char s_arena[BWPB][ARENA];
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
         const int64_t v0 = Q.base + u;
         const int64_t b0 = Q.rp[v0];
         const int d = (int)(Q.rp[v0 + 1] - b0);
-        if (d < Q.kind - 1 || d > 256) continue;
+        if (d < Q.kind - 1 || d > 128) continue;
         switch (Q.kind) {
           case 3: batch_root<3>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
           case 4: batch_root<4>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
@@ -253,12 +253,12 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
   hipLaunchKernelGGL(jk::batch_kernel, dim3(grid), dim3(64 * jk::BWPB), 0, st, a);
   prof_end(c, st);
   if (hipGetLastError() != hipSuccess) return JGPM_E_HIP;
-  // wide clique roots (out-degree > 256) of each query, into its slot
+  // wide clique roots (out-degree > 128) of each query, into its slot
   for (int i = 0; i < n; ++i) {
     if (hq[i].kind >= 100) continue;
     jgpm_graph_s* g = q[i].graph;
     int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;
-    rc = enqueue_clique(g, hq[i].kind, false, rb, re, 0, st, false, jk::TIER_LDS8, c->d_bcnt + i,
+    rc = enqueue_clique(g, hq[i].kind, false, rb, re, 0, st, false, jk::TIER_LDS4, c->d_bcnt + i,
                         c->d_bq32 + (size_t)jk::NUM_TIERS * i);
     if (rc) return rc;
   }

@@ -279,6 +279,22 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
   }
 }
 
+// Open-addressing hash of the root's members in LDS: slot of a vertex id.
+__device__ __forceinline__ int hslot(int32_t y, int hs) {
+  return (int)__umulhi((uint32_t)y * 2654435761u, (uint32_t)hs);
+}
+// Position stored for y, or -1 if y is not a member (linear probing; the
+// table always has an empty slot because hs > d).
+__device__ __forceinline__ int hfind(const int32_t* hk, const int16_t* hp, int hs, int32_t y) {
+  int h = hslot(y, hs);
+  for (;;) {
+    const int32_t k = hk[h];
+    if (k == y) return hp[h];
+    if (k < 0) return -1;
+    h = (h + 1 == hs) ? 0 : h + 1;
+  }
+}
+
 // LDS layout of one root slot.  The build-phase arrays (segment offsets,
 // list bases, long-list degrees) and the count-phase arrays (compacted row
 // lists, deferred rows, the compressed bitmap) share one region.
@@ -298,10 +314,12 @@ struct Lay {
   // build view
   static constexpr size_t B_OFFS = 0;
   static constexpr size_t B_UB = al(B_OFFS + 4 * (DM + 1));
+  static constexpr int HCAP = DM + DM / 4 + 1;                          // hash slots (load <= 0.8)
   static constexpr size_t B_HFL = al(B_UB + 8 * DM);                    // u32 [DM/32] long-list flags
-  static constexpr size_t B_KEY = al(B_HFL + 4 * ((DM + 31) / 32));      // int32 [DM] order keys
-  static constexpr size_t B_RNK = al(B_KEY + (MULTI ? 4 * DM : 0));      // int16 [DM] rank in DAG order
-  static constexpr size_t B_END = al(B_RNK + (MULTI ? 2 * DM : 0));
+  static constexpr size_t B_KEY = al(B_HFL + 4 * ((DM + 31) / 32));      // int32 [HCAP] order keys, then hash keys
+  static constexpr size_t B_RNK = al(B_KEY + 4 * HCAP);                  // int16 [DM] rank in DAG order
+  static constexpr size_t B_HPO = al(B_RNK + (MULTI ? 2 * DM : 0));      // int16 [HCAP] hash slot -> position
+  static constexpr size_t B_END = al(B_HPO + 2 * HCAP);
   // count view
   static constexpr size_t C_J = 0;                                          // int16 [G][DM]
   static constexpr size_t C_JB = al(C_J + (MULTI ? 2 * G * DM : 0));        // (unused)
@@ -337,6 +355,8 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   uint32_t* hfl = reinterpret_cast<uint32_t*>(U + LY::B_HFL);
   int32_t* key = reinterpret_cast<int32_t*>(U + LY::B_KEY);
   int16_t* rnk = reinterpret_cast<int16_t*>(U + LY::B_RNK);
+  int32_t* hk = key;  // after ranking, the key region holds the hash table
+  int16_t* hp = reinterpret_cast<int16_t*>(U + LY::B_HPO);
   // Multi-word roots number their members in DAG order (rank of ord[]), so
   // every row only has bits above its own position.
   constexpr bool RANK = LY::MULTI;
@@ -375,6 +395,16 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       wL[pos] = 16u + 4u * (uint32_t)(rp[L[t] + 1] - ub[t]);
     }
   }
+  // hash table of the members: id -> bit position (DAG-order rank when ranked)
+  const int hs = min(LY::HCAP, 2 * d + 1);
+  for (int t = r; t < hs; t += Grp<G>::size) hk[t] = -1;
+  Grp<G>::sync();
+  for (int t = r; t < d; t += Grp<G>::size) {
+    const int32_t y = L[t];
+    int h = hslot(y, hs);
+    while (atomicCAS(&hk[h], -1, y) != -1) h = (h + 1 == hs) ? 0 : h + 1;
+    hp[h] = (int16_t)(RANK ? (int)rnk[t] : t);
+  }
   if constexpr (NEED_ROWS) {
     for (int t = r; t < d * W; t += Grp<G>::size) rowp[t] = 0;
   }
@@ -387,13 +417,12 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   for (int f = r; f < T; f += Grp<G>::size) {
     int i = seg_search(off, d, f);
     int32_t y = ci[ub[i] + (f - off[i])];
-    int p = lower_bound_dev(L, d, y);
-    if (p < d && L[p] == y) {
+    const int rpb = hfind(hk, hp, hs, y);
+    if (rpb >= 0) {
       if constexpr (K == 3) {
         ++acc;
       } else {
         const int ri = RANK ? (int)rnk[i] : i;
-        const int rpb = RANK ? (int)rnk[p] : p;
         atomicOr((unsigned long long*)&rowp[ri * W + (rpb >> 6)], 1ull << (rpb & 63));
       }
     }
@@ -493,6 +522,58 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
           else acc += rec1<K - 3, false>(myrow, Rc, wL);
         }
       } else {
+        if constexpr (K == 5 && WMAX <= 16) {
+          // 5-cliques: triangles inside S.  One flat loop per lane over a
+          // work list shared through an LDS counter: a step either takes the
+          // next member j (candidates c = R[i] & R[j]) or evaluates one leaf
+          // x of c (|c & R[x]|); both are the same masked word loop, so lanes
+          // stay busy until the row's work is gone.
+          const int nw = whi - wlo + 1;
+          int* nxt = reinterpret_cast<int*>(Rc);
+          if (ln == 0) *nxt = 0;
+          __syncwarp();
+          uint64_t ri[WMAX], c[WMAX];
+#pragma unroll
+          for (int w = 0; w < WMAX; ++w) {
+            ri[w] = (w < nw) ? R(i, wlo + w) : 0ull;
+            c[w] = 0ull;
+          }
+          int t = atomicAdd(nxt, 1);
+          int cur = -1;       // current leaf bit in c, -1: take a new member
+          int y = t < n ? J[t] : 0;
+          bool grab = true;
+          while (__ballot(t < n) != 0ull) {
+            if (t < n) {
+              const int ws = ((y + 1) >> 6) - wlo;  // row y has no bits before its own position
+              uint64_t sum = 0;
+              uint64_t val[WMAX];
+#pragma unroll
+              for (int w = 0; w < WMAX; ++w) {
+                const uint64_t base = grab ? ri[w] : c[w];
+                val[w] = (w >= ws && w < nw && base) ? (base & R(y, wlo + w)) : 0ull;
+                sum += (uint64_t)__popcll(val[w]);
+              }
+              if (grab) {
+#pragma unroll
+                for (int w = 0; w < WMAX; ++w) c[w] = val[w];
+                if constexpr (BYTES) acc += wL[y];
+                cur = next_bit<WMAX>(c, nw, 0);
+              } else {
+                if constexpr (BYTES) acc += wL[y];
+                else acc += sum;
+                cur = next_bit<WMAX>(c, nw, cur + 1);
+              }
+              if (cur < 0) {
+                t = atomicAdd(nxt, 1);
+                grab = true;
+                y = t < n ? J[t] : 0;
+              } else {
+                grab = false;
+                y = wlo * 64 + cur;
+              }
+            }
+          }
+        } else
         // lanes over the members j of S; candidate set R[i] & R[j] held
         // in a register window of the row's nonzero words
         for (int t = ln; t < n; t += 64) {
