@@ -279,19 +279,16 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
   }
 }
 
-// Open-addressing hash of the root's members in LDS: slot of a vertex id.
-__device__ __forceinline__ int hslot(int32_t y, int hs) {
-  return (int)__umulhi((uint32_t)y * 2654435761u, (uint32_t)hs);
-}
-// Position stored for y, or -1 if y is not a member (linear probing; the
-// table always has an empty slot because hs > d).
-__device__ __forceinline__ int hfind(const int32_t* hk, const int16_t* hp, int hs, int32_t y) {
-  int h = hslot(y, hs);
+// Member table of 16-bit positions into L, power-of-two size with at least
+// four slots per member (load <= 1/4): Fibonacci hash, linear probing; the
+// stored position is confirmed against L.
+__device__ __forceinline__ int hfind16(const int16_t* h16, const int32_t* L, int mask, int shift, int32_t y) {
+  int h = (int)(((uint32_t)y * 2654435761u) >> shift);
   for (;;) {
-    const int32_t k = hk[h];
-    if (k == y) return hp[h];
-    if (k < 0) return -1;
-    h = (h + 1 == hs) ? 0 : h + 1;
+    const int t = h16[h];
+    if (t < 0) return -1;
+    if (L[t] == y) return t;
+    h = (h + 1) & mask;
   }
 }
 
@@ -314,12 +311,16 @@ struct Lay {
   // build view
   static constexpr size_t B_OFFS = 0;
   static constexpr size_t B_UB = al(B_OFFS + 4 * (DM + 1));
-  static constexpr int HCAP = DM + DM / 4 + 1;                          // hash slots (load <= 0.8)
+  // member lookup: binary search over L for one-word roots, else the
+  // 16-bit position hash with 4 * DM slots
+  static constexpr bool HASH = WMAX > 1;
+  static constexpr int HCAP = HASH ? 4 * DM : 0;
+  static constexpr size_t HBYTES = 2 * (size_t)HCAP;
+  static constexpr size_t KBYTES = MULTI ? 4 * (size_t)DM : 0;
   static constexpr size_t B_HFL = al(B_UB + 8 * DM);                    // u32 [DM/32] long-list flags
-  static constexpr size_t B_KEY = al(B_HFL + 4 * ((DM + 31) / 32));      // int32 [HCAP] order keys, then hash keys
-  static constexpr size_t B_RNK = al(B_KEY + 4 * HCAP);                  // int16 [DM] rank in DAG order
-  static constexpr size_t B_HPO = al(B_RNK + (MULTI ? 2 * DM : 0));      // int16 [HCAP] hash slot -> position
-  static constexpr size_t B_END = al(B_HPO + 2 * HCAP);
+  static constexpr size_t B_KEY = al(B_HFL + 4 * ((DM + 31) / 32));      // int32 [DM] order keys, then int16 [HCAP] hash
+  static constexpr size_t B_RNK = al(B_KEY + (HBYTES > KBYTES ? HBYTES : KBYTES));  // int16 [DM] rank in DAG order
+  static constexpr size_t B_END = al(B_RNK + (MULTI ? 2 * DM : 0));
   // count view
   static constexpr size_t C_J = 0;                                          // int16 [G][DM]
   static constexpr size_t C_JB = al(C_J + (MULTI ? 2 * G * DM : 0));        // (unused)
@@ -355,8 +356,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   uint32_t* hfl = reinterpret_cast<uint32_t*>(U + LY::B_HFL);
   int32_t* key = reinterpret_cast<int32_t*>(U + LY::B_KEY);
   int16_t* rnk = reinterpret_cast<int16_t*>(U + LY::B_RNK);
-  int32_t* hk = key;  // after ranking, the key region holds the hash table
-  int16_t* hp = reinterpret_cast<int16_t*>(U + LY::B_HPO);
+  int16_t* h16 = reinterpret_cast<int16_t*>(key);  // after ranking, the key region holds the hash table
   // Multi-word roots number their members in DAG order (rank of ord[]), so
   // every row only has bits above its own position.
   constexpr bool RANK = LY::MULTI;
@@ -395,15 +395,35 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       wL[pos] = 16u + 4u * (uint32_t)(rp[L[t] + 1] - ub[t]);
     }
   }
-  // hash table of the members: id -> bit position (DAG-order rank when ranked)
-  const int hs = min(LY::HCAP, 2 * d + 1);
-  for (int t = r; t < hs; t += Grp<G>::size) hk[t] = -1;
-  Grp<G>::sync();
-  for (int t = r; t < d; t += Grp<G>::size) {
-    const int32_t y = L[t];
-    int h = hslot(y, hs);
-    while (atomicCAS(&hk[h], -1, y) != -1) h = (h + 1 == hs) ? 0 : h + 1;
-    hp[h] = (int16_t)(RANK ? (int)rnk[t] : t);
+  // hash table of the members: slot -> position in L (power of two >= 4d
+  // slots, so probes stay short and an empty slot always exists)
+  int hs = 0, hshift = 0;
+  if constexpr (LY::HASH) {
+    int lg = 6;
+    while ((1 << lg) < 4 * d) ++lg;
+    hs = 1 << lg;
+    hshift = 32 - lg;
+    uint32_t* hw = reinterpret_cast<uint32_t*>(h16);
+    for (int t = r; t < (hs >> 1); t += Grp<G>::size) hw[t] = 0xffffffffu;
+    Grp<G>::sync();
+    for (int t = r; t < d; t += Grp<G>::size) {
+      // two 16-bit slots per word: claim the first empty one by CAS on the word
+      int h = (int)(((uint32_t)L[t] * 2654435761u) >> hshift);
+      volatile uint32_t* vw = hw;
+      uint32_t old = vw[h >> 1];
+      for (;;) {
+        const int sh = (h & 1) * 16;
+        if (((old >> sh) & 0xffffu) != 0xffffu) {
+          h = (h + 1) & (hs - 1);
+          old = vw[h >> 1];
+          continue;
+        }
+        const uint32_t nv = (old & ~(0xffffu << sh)) | ((uint32_t)t << sh);
+        const uint32_t prev = atomicCAS(&hw[h >> 1], old, nv);
+        if (prev == old) break;
+        old = prev;
+      }
+    }
   }
   if constexpr (NEED_ROWS) {
     for (int t = r; t < d * W; t += Grp<G>::size) rowp[t] = 0;
@@ -417,7 +437,14 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   for (int f = r; f < T; f += Grp<G>::size) {
     int i = seg_search(off, d, f);
     int32_t y = ci[ub[i] + (f - off[i])];
-    const int rpb = hfind(hk, hp, hs, y);
+    int rpb;  // bit position of y in the rows, -1 if y is not a member
+    if constexpr (LY::HASH) {
+      rpb = hfind16(h16, L, hs - 1, hshift, y);
+      if (RANK && rpb >= 0) rpb = rnk[rpb];
+    } else {
+      const int p = lower_bound_dev(L, d, y);
+      rpb = (p < d && L[p] == y) ? p : -1;
+    }
     if (rpb >= 0) {
       if constexpr (K == 3) {
         ++acc;
