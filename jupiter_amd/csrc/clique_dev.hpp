@@ -279,6 +279,67 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
   }
 }
 
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+typedef int v16i32 __attribute__((ext_vector_type(16)));
+
+// 16 bits -> 16 bytes of 0/1 (bit e -> byte e): each nibble is spread by one
+// 24-bit multiply (n * 0x204081 puts bit k at 8k, no carries) and masked.
+__device__ __forceinline__ v4i32 bits_to_bytes(uint32_t b) {
+  v4i32 v;
+  v[0] = (int)(((b & 15u) * 0x00204081u) & 0x01010101u);
+  v[1] = (int)((((b >> 4) & 15u) * 0x00204081u) & 0x01010101u);
+  v[2] = (int)((((b >> 8) & 15u) * 0x00204081u) & 0x01010101u);
+  v[3] = (int)((((b >> 12) & 15u) * 0x00204081u) & 0x01010101u);
+  return v;
+}
+
+// Triangles of the DAG induced on S = R[i] (rows ranked in DAG order, so
+// every row only has bits above its own position), by int8 matrix cores:
+//   sum over member pairs x -> y of |R[y] ∩ R[i] ∩ R[x]|
+// = sum over 32x32 blocks (X <= Y) of  mask(x -> y) . (A_Y B_X^T)[y][x],
+// where A_Y holds the rows R[y] ∩ R[i] of the Y block's members and B_X the
+// rows R[x] of the X block's members, both over the row's word window in
+// chunks of 32 positions (one v_mfma_i32_32x32x32_i8 per chunk).  Lane l
+// supplies member 32B + l%32 and the 16 positions 16 * (l / 32) .. +15 of
+// the chunk; accumulator element a of lane l is entry
+// [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  J lists S's positions in
+// ascending order (n > 0 entries); every lane of the wave must call this.
+template <class RowT>
+__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n, int i, int wlo, int whi) {
+  const int ln = lane_id();
+  const int col = ln & 31, half = ln >> 5;
+  const int T = (n + 31) >> 5;
+  const int qe = 2 * whi + 1;
+  uint64_t s = 0;
+  for (int Y = 0; Y < T; ++Y) {
+    const int ay = 32 * Y + col;
+    const int y = ay < n ? (int)J[ay] : -1;
+    const int q0 = max(2 * wlo, ((int)J[32 * Y] + 1) >> 5);  // R[y] is empty up to the block's first member
+    for (int X = 0; X <= Y; ++X) {
+      const int bx = 32 * X + col;
+      const int x = bx < n ? (int)J[bx] : -1;
+      v16i32 acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = q0; q <= qe; ++q) {
+        const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
+        const uint32_t ab = y >= 0 ? (uint32_t)(((R(y, w) & R(i, w)) >> sh) & 0xffffull) : 0u;
+        const uint32_t bb = x >= 0 ? (uint32_t)((R(x, w) >> sh) & 0xffffull) : 0u;
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(ab), bits_to_bytes(bb), acc, 0, 0, 0);
+      }
+      if (x >= 0) {
+#pragma unroll
+        for (int a = 0; a < 16; ++a) {
+          const int ym = 32 * Y + 8 * (a >> 2) + 4 * half + (a & 3);
+          if (ym < n) {
+            const int yy = J[ym];
+            if ((R(x, yy >> 6) >> (yy & 63)) & 1ull) s += (uint64_t)(uint32_t)acc[a];
+          }
+        }
+      }
+    }
+  }
+  return s;
+}
+
 // Member table of 16-bit positions into L, power-of-two size with at least
 // four slots per member (load <= 1/4): Fibonacci hash, linear probing; the
 // stored position is confirmed against L.
@@ -549,7 +610,9 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
           else acc += rec1<K - 3, false>(myrow, Rc, wL);
         }
       } else {
-        if constexpr (K == 5 && WMAX <= 16) {
+        if constexpr (K == 5 && !BYTES) {
+          acc += tri_mfma(R, J, n, i, wlo, whi);
+        } else if constexpr (K == 5 && WMAX <= 16) {
           // 5-cliques: triangles inside S.  One flat loop per lane over a
           // work list shared through an LDS counter: a step either takes the
           // next member j (candidates c = R[i] & R[j]) or evaluates one leaf
