@@ -44,6 +44,22 @@ __device__ __forceinline__ int seg_search(const int* a, int n, int f) {
   return lo;
 }
 
+// Cursor form of seg_search for increasing f: a[i] <= f on entry; returns
+// the last j >= i with a[j] <= f (two steps forward, then a binary search).
+__device__ __forceinline__ int seg_advance(const int* a, int n, int i, int f) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (a[i + 1] > f) return i;
+    ++i;
+  }
+  int lo = i, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] <= f) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // first index p in [0, n) with a[p] >= y (n if none)
 template <class T>
 __device__ __forceinline__ int lower_bound_dev(const T* a, int n, int y) {
@@ -494,24 +510,46 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   if constexpr (K == 3 && BYTES) return;  // TC work model needs no build
   group_excl_scan<G>(off, d, s_wtot);
   const int T = off[d];
-  // ---- flattened stream of the light lists
-  for (int f = r; f < T; f += Grp<G>::size) {
-    int i = seg_search(off, d, f);
-    int32_t y = ci[ub[i] + (f - off[i])];
-    int rpb;  // bit position of y in the rows, -1 if y is not a member
-    if constexpr (LY::HASH) {
-      rpb = hfind16(h16, L, hs - 1, hshift, y);
-      if (RANK && rpb >= 0) rpb = rnk[rpb];
-    } else {
-      const int p = lower_bound_dev(L, d, y);
-      rpb = (p < d && L[p] == y) ? p : -1;
-    }
-    if (rpb >= 0) {
-      if constexpr (K == 3) {
-        ++acc;
-      } else {
-        const int ri = RANK ? (int)rnk[i] : i;
-        atomicOr((unsigned long long*)&rowp[ri * W + (rpb >> 6)], 1ull << (rpb & 63));
+  // ---- flattened stream of the light lists: thread r takes elements
+  // r, r + NT, ... (consecutive lanes read consecutive addresses); a thread's
+  // segment index only moves forward, and four loads are in flight at once
+  {
+    constexpr int NT = Grp<G>::size;
+    constexpr int UF = 4;
+    int i = r < T ? seg_search(off, d, r) : 0;
+    for (int f0 = r; f0 < T; f0 += UF * NT) {
+      int32_t yv[UF];
+      int iv[UF];
+#pragma unroll
+      for (int u = 0; u < UF; ++u) {
+        const int f = f0 + u * NT;
+        iv[u] = -1;
+        if (f < T) {
+          i = seg_advance(off, d, i, f);
+          iv[u] = i;
+          yv[u] = ci[ub[i] + (f - off[i])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UF; ++u) {
+        if (iv[u] < 0) continue;
+        const int32_t y = yv[u];
+        int rpb;  // bit position of y in the rows, -1 if y is not a member
+        if constexpr (LY::HASH) {
+          rpb = hfind16(h16, L, hs - 1, hshift, y);
+          if (RANK && rpb >= 0) rpb = rnk[rpb];
+        } else {
+          const int p = lower_bound_dev(L, d, y);
+          rpb = (p < d && L[p] == y) ? p : -1;
+        }
+        if (rpb >= 0) {
+          if constexpr (K == 3) {
+            ++acc;
+          } else {
+            const int ri = RANK ? (int)rnk[iv[u]] : iv[u];
+            atomicOr((unsigned long long*)&rowp[ri * W + (rpb >> 6)], 1ull << (rpb & 63));
+          }
+        }
       }
     }
   }
