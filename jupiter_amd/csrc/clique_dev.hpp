@@ -295,6 +295,9 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
   }
 }
 
+#ifndef JGPM_TILE_MIN
+#define JGPM_TILE_MIN 16
+#endif
 typedef int v4i32 __attribute__((ext_vector_type(4)));
 typedef int v16i32 __attribute__((ext_vector_type(16)));
 
@@ -320,8 +323,49 @@ __device__ __forceinline__ v4i32 bits_to_bytes(uint32_t b) {
 // the chunk; accumulator element a of lane l is entry
 // [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  J lists S's positions in
 // ascending order (n > 0 entries); every lane of the wave must call this.
+// Matrix-core instructions run on the whole wave whatever EXEC holds, so
+// every condition that guards one must be wave-uniform and scalar (a real
+// branch), never a lane mask: values the compiler cannot prove uniform are
+// passed through readfirstlane.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ v16i32 zero16() {
+  v16i32 z;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) z[k] = 0;
+  return z;
+}
+
+// 16 positions of row `row` in 32-position chunk q, the half given by sh.
 template <class RowT>
-__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n, int i, int wlo, int whi) {
+__device__ __forceinline__ uint32_t row_slice(const RowT& R, int row, int w, int sh) {
+  return row >= 0 ? (uint32_t)((R(row, w) >> sh) & 0xffffull) : 0u;
+}
+
+// One accumulator per block pair (the form for narrow roots, where the
+// register budget of the tiled form would cost occupancy).
+// Operand row with a single 1 at position y (when y lies in the 16
+// positions of chunk q's half `half`): the A side of E_Y * R_X^T, whose entry
+// [y][x] is bit y of R[x], i.e. the edge x -> y.
+__device__ __forceinline__ v4i32 one_hot16(int y, int q, int half) {
+  v4i32 v;
+  const bool in = y >= 0 && (y >> 4) == 2 * q + half;
+  const int e = y & 15;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = (in && (e >> 2) == k) ? (1 << (8 * (e & 3))) : 0;
+  return v;
+}
+
+__device__ __forceinline__ uint64_t dot16(const v16i32& c, const v16i32& m) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int a = 0; a < 16; ++a) s += (uint32_t)c[a] * (uint32_t)m[a];
+  return s;
+}
+
+template <class RowT>
+__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_) {
+  const int n = uni(n_), i = uni(i_), wlo = uni(wlo_), whi = uni(whi_);
   const int ln = lane_id();
   const int col = ln & 31, half = ln >> 5;
   const int T = (n + 31) >> 5;
@@ -330,26 +374,87 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n, int i, int 
   for (int Y = 0; Y < T; ++Y) {
     const int ay = 32 * Y + col;
     const int y = ay < n ? (int)J[ay] : -1;
-    const int q0 = max(2 * wlo, ((int)J[32 * Y] + 1) >> 5);  // R[y] is empty up to the block's first member
+    const int jf = uni(J[32 * Y]), jl = uni(J[min(n - 1, 32 * Y + 31)]);
+    const int ma = jf >> 5, mb = jl >> 5;  // chunks of the block's members
+    const int q0 = min(ma, max(2 * wlo, (jf + 1) >> 5));
     for (int X = 0; X <= Y; ++X) {
       const int bx = 32 * X + col;
       const int x = bx < n ? (int)J[bx] : -1;
-      v16i32 acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      v16i32 acc = zero16(), msk = zero16();
       for (int q = q0; q <= qe; ++q) {
         const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
         const uint32_t ab = y >= 0 ? (uint32_t)(((R(y, w) & R(i, w)) >> sh) & 0xffffull) : 0u;
-        const uint32_t bb = x >= 0 ? (uint32_t)((R(x, w) >> sh) & 0xffffull) : 0u;
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(ab), bits_to_bytes(bb), acc, 0, 0, 0);
+        const v4i32 bo = bits_to_bytes(row_slice(R, x, w, sh));
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(ab), bo, acc, 0, 0, 0);
+        if (q >= ma && q <= mb) msk = __builtin_amdgcn_mfma_i32_32x32x32_i8(one_hot16(y, q, half), bo, msk, 0, 0, 0);
       }
-      if (x >= 0) {
-#pragma unroll
-        for (int a = 0; a < 16; ++a) {
-          const int ym = 32 * Y + 8 * (a >> 2) + 4 * half + (a & 3);
-          if (ym < n) {
-            const int yy = J[ym];
-            if ((R(x, yy >> 6) >> (yy & 63)) & 1ull) s += (uint64_t)(uint32_t)acc[a];
+      s += dot16(acc, msk);
+    }
+  }
+  return s;
+}
+
+template <class RowT>
+__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_) {
+  const int n = uni(n_), i = uni(i_), wlo = uni(wlo_), whi = uni(whi_);
+  const int ln = lane_id();
+  const int col = ln & 31, half = ln >> 5;
+  const int T = (n + 31) >> 5;
+  const int qe = 2 * whi + 1;
+  auto member = [&](int B) -> int {
+    const int a = 32 * B + col;
+    return (B < T && a < n) ? (int)J[a] : -1;
+  };
+  uint64_t s = 0;
+  // 2 x 2 tiles of block pairs: each chunk's four operand slices feed four
+  // independent count accumulators and, on the chunks holding the Y blocks'
+  // own members, four edge-mask accumulators
+  for (int Y0 = 0; Y0 < T; Y0 += 2) {
+    const int Y1 = Y0 + 1;
+    const bool hy1 = Y1 < T;
+    const int y0 = member(Y0), y1 = member(Y1);
+    const int jf0 = uni(J[32 * Y0]);
+    const int ma0 = jf0 >> 5, mb0 = uni(J[min(n - 1, 32 * Y0 + 31)]) >> 5;
+    const int ma1 = hy1 ? uni(J[32 * Y1]) >> 5 : 1 << 30, mb1 = hy1 ? uni(J[min(n - 1, 32 * Y1 + 31)]) >> 5 : -1;
+    const int q0 = min(ma0, max(2 * wlo, (jf0 + 1) >> 5));  // rows of the tile are empty before this chunk
+    const int Xend = hy1 ? Y1 : Y0;
+    for (int X0 = 0; X0 <= Xend; X0 += 2) {
+      const int X1 = X0 + 1;
+      const bool hx1 = X1 <= Xend;
+      const bool v01 = hx1 && X1 <= Y0;  // block pair (Y0, X1) has edges x -> y only if X1 <= Y0
+      const int x0 = member(X0), x1 = hx1 ? member(X1) : -1;
+      v16i32 c00 = zero16(), c01 = zero16(), c10 = zero16(), c11 = zero16();
+      v16i32 m00 = zero16(), m01 = zero16(), m10 = zero16(), m11 = zero16();
+      for (int q = q0; q <= qe; ++q) {
+        const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
+        const uint64_t ri = R(i, w);
+        const bool in0 = q >= ma0 && q <= mb0, in1 = q >= ma1 && q <= mb1;
+        const v4i32 a0 = bits_to_bytes(y0 >= 0 ? (uint32_t)(((R(y0, w) & ri) >> sh) & 0xffffull) : 0u);
+        const v4i32 b0 = bits_to_bytes(row_slice(R, x0, w, sh));
+        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+        const v4i32 b1 = bits_to_bytes(row_slice(R, x1, w, sh));  // zero when the tile has no X1 block
+        if (v01) c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
+        if (in0) {
+          const v4i32 e0 = one_hot16(y0, q, half);
+          m00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e0, b0, m00, 0, 0, 0);
+          if (v01) m01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e0, b1, m01, 0, 0, 0);
+        }
+        if (hy1) {
+          const v4i32 a1 = bits_to_bytes(y1 >= 0 ? (uint32_t)(((R(y1, w) & ri) >> sh) & 0xffffull) : 0u);
+          c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
+          if (hx1) c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+          if (in1) {
+            const v4i32 e1 = one_hot16(y1, q, half);
+            m10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e1, b0, m10, 0, 0, 0);
+            if (hx1) m11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e1, b1, m11, 0, 0, 0);
           }
         }
+      }
+      s += dot16(c00, m00);
+      if (v01) s += dot16(c01, m01);
+      if (hy1) {
+        s += dot16(c10, m10);
+        if (hx1) s += dot16(c11, m11);
       }
     }
   }
@@ -649,7 +754,8 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         }
       } else {
         if constexpr (K == 5 && !BYTES) {
-          acc += tri_mfma(R, J, n, i, wlo, whi);
+          if constexpr (WMAX >= JGPM_TILE_MIN) acc += tri_mfma(R, J, n, i, wlo, whi);
+          else acc += tri_mfma1(R, J, n, i, wlo, whi);
         } else if constexpr (K == 5 && WMAX <= 16) {
           // 5-cliques: triangles inside S.  One flat loop per lane over a
           // work list shared through an LDS counter: a step either takes the
