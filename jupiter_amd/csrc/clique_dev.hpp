@@ -183,15 +183,27 @@ __device__ uint64_t recW(const RowT& R, int wlo, int whi, const int* p, const ui
   return s;
 }
 
-struct RowsLDS {
-  uint64_t* p;
-  int W;
-  __device__ uint64_t operator()(int i, int w) const { return p[i * W + w]; }
+// Word w of bitmap row i sits at i * S + (w ^ ((i >> sh) & X)): S = W when
+// W is odd or a power of two, else W + 1 (odd); for a power of two W the
+// word is XOR-swizzled by the row bits above the 32 / W rows that one pass
+// of the stride spreads over 32 eight-byte bank slots.  Either way lanes that
+// read the same word of 32 consecutive rows hit 32 different slots.  d * S
+// never exceeds 64 * WMAX * WMAX, the rows' allocation.
+struct RowLayout {
+  int S, X, sh;
+  __device__ explicit RowLayout(int W) {  // W >= 0 (W == 0: a root without members)
+    const bool p2 = W > 0 && (W & (W - 1)) == 0;
+    S = ((W & 1) || p2) ? W : W + 1;
+    X = p2 ? W - 1 : 0;
+    sh = 0;
+    while (sh < 5 && (W << sh) < 32) ++sh;
+  }
+  __device__ int operator()(int i, int w) const { return i * S + (w ^ ((i >> sh) & X)); }
 };
 struct RowsGlobal {
   uint64_t* p;
-  int W;
-  __device__ uint64_t operator()(int i, int w) const { return __builtin_nontemporal_load(p + (int64_t)i * W + w); }
+  RowLayout lay;
+  __device__ uint64_t operator()(int i, int w) const { return __builtin_nontemporal_load(p + lay(i, w)); }
 };
 
 // Block reduction of a per-thread uint64 and one atomicAdd per block.
@@ -295,34 +307,31 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
   }
 }
 
-#ifndef JGPM_TILE_MIN
-#define JGPM_TILE_MIN 16
-#endif
 typedef int v4i32 __attribute__((ext_vector_type(4)));
 typedef int v16i32 __attribute__((ext_vector_type(16)));
 
-// 16 bits -> 16 bytes of 0/1 (bit e -> byte e): each nibble is spread by one
-// 24-bit multiply (n * 0x204081 puts bit k at 8k, no carries) and masked.
-__device__ __forceinline__ v4i32 bits_to_bytes(uint32_t b) {
+// Byte -> 8 bytes of 0/1 (bit e -> byte e), the table behind bits_to_bytes:
+// each nibble is spread by one 24-bit multiply (n * 0x204081 puts bit k at
+// 8k, no carries) and masked.  Threads t, t + nt, ... of the block fill it.
+__device__ __forceinline__ void fill_bit_lut(uint64_t* lut, int t, int nt) {
+  for (int c = t; c < 256; c += nt) {
+    const uint64_t lo = ((uint32_t)(c & 15) * 0x00204081u) & 0x01010101u;
+    const uint64_t hi = ((uint32_t)(c >> 4) * 0x00204081u) & 0x01010101u;
+    lut[c] = lo | (hi << 32);
+  }
+}
+
+// 16 bits -> 16 bytes of 0/1 (bit e -> byte e): two lookups in the LDS table.
+__device__ __forceinline__ v4i32 bits_to_bytes(const uint64_t* lut, uint32_t b) {
+  const uint64_t lo = lut[b & 255u], hi = lut[(b >> 8) & 255u];
   v4i32 v;
-  v[0] = (int)(((b & 15u) * 0x00204081u) & 0x01010101u);
-  v[1] = (int)((((b >> 4) & 15u) * 0x00204081u) & 0x01010101u);
-  v[2] = (int)((((b >> 8) & 15u) * 0x00204081u) & 0x01010101u);
-  v[3] = (int)((((b >> 12) & 15u) * 0x00204081u) & 0x01010101u);
+  v[0] = (int)(uint32_t)lo;
+  v[1] = (int)(uint32_t)(lo >> 32);
+  v[2] = (int)(uint32_t)hi;
+  v[3] = (int)(uint32_t)(hi >> 32);
   return v;
 }
 
-// Triangles of the DAG induced on S = R[i] (rows ranked in DAG order, so
-// every row only has bits above its own position), by int8 matrix cores:
-//   sum over member pairs x -> y of |R[y] ∩ R[i] ∩ R[x]|
-// = sum over 32x32 blocks (X <= Y) of  mask(x -> y) . (A_Y B_X^T)[y][x],
-// where A_Y holds the rows R[y] ∩ R[i] of the Y block's members and B_X the
-// rows R[x] of the X block's members, both over the row's word window in
-// chunks of 32 positions (one v_mfma_i32_32x32x32_i8 per chunk).  Lane l
-// supplies member 32B + l%32 and the 16 positions 16 * (l / 32) .. +15 of
-// the chunk; accumulator element a of lane l is entry
-// [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  J lists S's positions in
-// ascending order (n > 0 entries); every lane of the wave must call this.
 // Matrix-core instructions run on the whole wave whatever EXEC holds, so
 // every condition that guards one must be wave-uniform and scalar (a real
 // branch), never a lane mask: values the compiler cannot prove uniform are
@@ -336,14 +345,31 @@ __device__ __forceinline__ v16i32 zero16() {
   return z;
 }
 
+// Sum of the accumulator entries [y][x] of block Y against column member x
+// whose pair is a DAG edge x -> y (bit y of R[x]), read from the rows.
+template <class RowT>
+__device__ __forceinline__ uint64_t edge_masked_sum(const RowT& R, const int16_t* J, int n, int Y, int x, int half,
+                                                    const v16i32& acc) {
+  uint64_t s = 0;
+  if (x >= 0) {
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int ym = 32 * Y + 8 * (a >> 2) + 4 * half + (a & 3);
+      if (ym < n) {
+        const int yy = J[ym];
+        if ((R(x, yy >> 6) >> (yy & 63)) & 1ull) s += (uint64_t)(uint32_t)acc[a];
+      }
+    }
+  }
+  return s;
+}
+
 // 16 positions of row `row` in 32-position chunk q, the half given by sh.
 template <class RowT>
 __device__ __forceinline__ uint32_t row_slice(const RowT& R, int row, int w, int sh) {
   return row >= 0 ? (uint32_t)((R(row, w) >> sh) & 0xffffull) : 0u;
 }
 
-// One accumulator per block pair (the form for narrow roots, where the
-// register budget of the tiled form would cost occupancy).
 // Operand row with a single 1 at position y (when y lies in the 16
 // positions of chunk q's half `half`): the A side of E_Y * R_X^T, whose entry
 // [y][x] is bit y of R[x], i.e. the edge x -> y.
@@ -363,8 +389,22 @@ __device__ __forceinline__ uint64_t dot16(const v16i32& c, const v16i32& m) {
   return s;
 }
 
+// Triangles of the DAG induced on S = R[i] (rows ranked in DAG order, so
+// every row only has bits above its own position), by int8 matrix cores:
+//   sum over member pairs x -> y of |R[y] ∩ R[i] ∩ R[x]|
+// = sum over 32x32 blocks (X <= Y) of  mask(x -> y) . (A_Y B_X^T)[y][x],
+// where A_Y holds the rows R[y] ∩ R[i] of the Y block's members and B_X the
+// rows R[x] of the X block's members, both over the row's word window in
+// chunks of 32 positions (one v_mfma_i32_32x32x32_i8 per chunk).  Lane l
+// supplies member 32B + l%32 and the 16 positions 16 * (l / 32) .. +15 of
+// the chunk; accumulator element a of lane l is entry
+// [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  J lists S's positions in
+// ascending order (n > 0 entries); every lane of the wave must call this.
+// This form keeps one accumulator per block pair (narrow roots, where the
+// registers of the tiled form below would cost occupancy).
 template <class RowT>
-__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_) {
+__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_,
+                          const uint64_t* lut) {
   const int n = uni(n_), i = uni(i_), wlo = uni(wlo_), whi = uni(whi_);
   const int ln = lane_id();
   const int col = ln & 31, half = ln >> 5;
@@ -374,28 +414,30 @@ __device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, i
   for (int Y = 0; Y < T; ++Y) {
     const int ay = 32 * Y + col;
     const int y = ay < n ? (int)J[ay] : -1;
-    const int jf = uni(J[32 * Y]), jl = uni(J[min(n - 1, 32 * Y + 31)]);
-    const int ma = jf >> 5, mb = jl >> 5;  // chunks of the block's members
-    const int q0 = min(ma, max(2 * wlo, (jf + 1) >> 5));
+    const int jf = uni(J[32 * Y]);
+    const int q0 = max(2 * wlo, (jf + 1) >> 5);  // R[y] is empty up to the block's first member
     for (int X = 0; X <= Y; ++X) {
       const int bx = 32 * X + col;
       const int x = bx < n ? (int)J[bx] : -1;
-      v16i32 acc = zero16(), msk = zero16();
+      v16i32 acc = zero16();
       for (int q = q0; q <= qe; ++q) {
         const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
         const uint32_t ab = y >= 0 ? (uint32_t)(((R(y, w) & R(i, w)) >> sh) & 0xffffull) : 0u;
-        const v4i32 bo = bits_to_bytes(row_slice(R, x, w, sh));
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(ab), bo, acc, 0, 0, 0);
-        if (q >= ma && q <= mb) msk = __builtin_amdgcn_mfma_i32_32x32x32_i8(one_hot16(y, q, half), bo, msk, 0, 0, 0);
+        const v4i32 bo = bits_to_bytes(lut, row_slice(R, x, w, sh));
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lut, ab), bo, acc, 0, 0, 0);
       }
-      s += dot16(acc, msk);
+      s += edge_masked_sum(R, J, n, Y, x, half, acc);
     }
   }
   return s;
 }
 
+// The same count with 2 x 2 tiles of block pairs (widest roots); the edge
+// mask is the product E_Y R_X^T of one-hot rows, accumulated on the chunks
+// that hold the Y block's members.
 template <class RowT>
-__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_) {
+__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_,
+                         const uint64_t* lut) {
   const int n = uni(n_), i = uni(i_), wlo = uni(wlo_), whi = uni(whi_);
   const int ln = lane_id();
   const int col = ln & 31, half = ln >> 5;
@@ -429,10 +471,10 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, in
         const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
         const uint64_t ri = R(i, w);
         const bool in0 = q >= ma0 && q <= mb0, in1 = q >= ma1 && q <= mb1;
-        const v4i32 a0 = bits_to_bytes(y0 >= 0 ? (uint32_t)(((R(y0, w) & ri) >> sh) & 0xffffull) : 0u);
-        const v4i32 b0 = bits_to_bytes(row_slice(R, x0, w, sh));
+        const v4i32 a0 = bits_to_bytes(lut, y0 >= 0 ? (uint32_t)(((R(y0, w) & ri) >> sh) & 0xffffull) : 0u);
+        const v4i32 b0 = bits_to_bytes(lut, row_slice(R, x0, w, sh));
         c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-        const v4i32 b1 = bits_to_bytes(row_slice(R, x1, w, sh));  // zero when the tile has no X1 block
+        const v4i32 b1 = bits_to_bytes(lut, row_slice(R, x1, w, sh));  // zero when the tile has no X1 block
         if (v01) c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
         if (in0) {
           const v4i32 e0 = one_hot16(y0, q, half);
@@ -440,7 +482,7 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, in
           if (v01) m01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e0, b1, m01, 0, 0, 0);
         }
         if (hy1) {
-          const v4i32 a1 = bits_to_bytes(y1 >= 0 ? (uint32_t)(((R(y1, w) & ri) >> sh) & 0xffffull) : 0u);
+          const v4i32 a1 = bits_to_bytes(lut, y1 >= 0 ? (uint32_t)(((R(y1, w) & ri) >> sh) & 0xffffull) : 0u);
           c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
           if (hx1) c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
           if (in1) {
@@ -519,11 +561,12 @@ struct Lay {
 // Process one root v0 (list N+(v0) = ci[b0 .. b0 + d)) with a group of G
 // waves: build the bitmap rows in `rowp` and add the root's k-cliques (or,
 // with BYTES, its work-model bytes) to `acc`.  `ar` is the group's LDS
-// arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes.
+// arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes; `lut` is the block's
+// fill_bit_lut table (read by the 5-clique matrix-core count only).
 template <int G, int WMAX, int K, bool BYTES, bool LROWS>
 __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                              const int32_t* __restrict__ ord, int64_t b0, int d, int ratio, unsigned char* ar,
-                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc) {
+                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc, const uint64_t* lut) {
   using LY = Lay<G, WMAX, K, BYTES, LROWS>;
   constexpr int DM = LY::DM;
   constexpr bool NEED_ROWS = LY::ROWS;
@@ -543,6 +586,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   // every row only has bits above its own position.
   constexpr bool RANK = LY::MULTI;
   const int W = (d + 63) >> 6;
+  const RowLayout lay(W);
   if constexpr (BYTES) {
     if (r == 0) acc += 16 + 4 * (uint64_t)d;
   }
@@ -608,7 +652,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     }
   }
   if constexpr (NEED_ROWS) {
-    for (int t = r; t < d * W; t += Grp<G>::size) rowp[t] = 0;
+    for (int t = r; t < d * lay.S; t += Grp<G>::size) rowp[t] = 0;
   }
   if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   Grp<G>::sync();
@@ -617,10 +661,10 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   const int T = off[d];
   // ---- flattened stream of the light lists: thread r takes elements
   // r, r + NT, ... (consecutive lanes read consecutive addresses); a thread's
-  // segment index only moves forward, and four loads are in flight at once
+  // segment index only moves forward, and eight loads are in flight at once
   {
     constexpr int NT = Grp<G>::size;
-    constexpr int UF = 4;
+    constexpr int UF = 8;
     int i = r < T ? seg_search(off, d, r) : 0;
     for (int f0 = r; f0 < T; f0 += UF * NT) {
       int32_t yv[UF];
@@ -652,7 +696,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
             ++acc;
           } else {
             const int ri = RANK ? (int)rnk[iv[u]] : iv[u];
-            atomicOr((unsigned long long*)&rowp[ri * W + (rpb >> 6)], 1ull << (rpb & 63));
+            atomicOr((unsigned long long*)&rowp[lay(ri, rpb >> 6)], 1ull << (rpb & 63));
           }
         }
       }
@@ -673,11 +717,11 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       } else if constexpr (RANK) {
         if (f) {
           const int rt = rnk[t];
-          atomicOr((unsigned long long*)&rowp[ri * W + (rt >> 6)], 1ull << (rt & 63));
+          atomicOr((unsigned long long*)&rowp[lay(ri, rt >> 6)], 1ull << (rt & 63));
         }
       } else {
         uint64_t m = __ballot(f);
-        if (ln == 0 && m) atomicOr((unsigned long long*)&rowp[ri * W + (t0 >> 6)], m);
+        if (ln == 0 && m) atomicOr((unsigned long long*)&rowp[lay(ri, t0 >> 6)], m);
       }
     }
   }
@@ -695,8 +739,8 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     }
   } else {
     auto R = [&](int i, int w) -> uint64_t {
-      if constexpr (LROWS) return rowp[i * W + w];
-      else return RowsGlobal{rowp, W}(i, w);
+      if constexpr (LROWS) return rowp[lay(i, w)];
+      else return RowsGlobal{rowp, lay}(i, w);
     };
     int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;
     uint64_t* Rc = reinterpret_cast<uint64_t*>(U + LY::C_SC) + (size_t)wv * 64;
@@ -754,8 +798,9 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         }
       } else {
         if constexpr (K == 5 && !BYTES) {
-          if constexpr (WMAX >= JGPM_TILE_MIN) acc += tri_mfma(R, J, n, i, wlo, whi);
-          else acc += tri_mfma1(R, J, n, i, wlo, whi);
+          // tiles of block pairs pay for their registers only on the widest roots
+          if constexpr (WMAX >= 16) acc += tri_mfma(R, J, n, i, wlo, whi, lut);
+          else acc += tri_mfma1(R, J, n, i, wlo, whi, lut);
         } else if constexpr (K == 5 && WMAX <= 16) {
           // 5-cliques: triangles inside S.  One flat loop per lane over a
           // work list shared through an LDS counter: a step either takes the
@@ -824,9 +869,9 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
             int p2[2] = {i, j};
             if constexpr (BYTES) {
               acc += wL[j];
-              acc += recW<K - 4, 2, true>(RowsGlobal{rowp, W}, wlo, whi, p2, wL);
+              acc += recW<K - 4, 2, true>(RowsGlobal{rowp, lay}, wlo, whi, p2, wL);
             } else {
-              acc += recW<K - 3, 2, false>(RowsGlobal{rowp, W}, wlo, whi, p2, wL);
+              acc += recW<K - 3, 2, false>(RowsGlobal{rowp, lay}, wlo, whi, p2, wL);
             }
           }
         }

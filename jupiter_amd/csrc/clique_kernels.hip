@@ -46,6 +46,12 @@ clique_kernel(CliqueArgs a) {
   __shared__ __attribute__((aligned(16))) unsigned char s_arena[WPB][LY::SLOT];
   __shared__ int32_t s_wtot[G + 1];
   __shared__ int64_t s_task;
+  constexpr bool LUT = K == 5 && !BYTES && WMAX > 1;  // the matrix-core count expands bits through a table
+  __shared__ uint64_t s_lut[LUT ? 256 : 1];
+  if constexpr (LUT) {
+    fill_bit_lut(s_lut, (int)threadIdx.x, NT);
+    __syncthreads();
+  }
   const int slot = (G == 1) ? (int)(threadIdx.x >> 6) : 0;
   const int ln = lane_id();
   unsigned char* ar = s_arena[slot];
@@ -72,7 +78,7 @@ clique_kernel(CliqueArgs a) {
       const int64_t b0 = rp[v0];
       const int d = (int)(rp[v0 + 1] - b0);
       if (d < a.dmin || d > a.dmax || d > DM) continue;
-      process_root<G, WMAX, K, BYTES, LROWS>(rp, a.ci, a.ord, b0, d, a.ratio, ar, rowp, s_wtot, acc);
+      process_root<G, WMAX, K, BYTES, LROWS>(rp, a.ci, a.ord, b0, d, a.ratio, ar, rowp, s_wtot, acc, s_lut);
     }
   }
   block_add<NT>(acc, a.out);
