@@ -11,7 +11,9 @@ Also reported on the same JSON line:
                 region, from HIP events on the library's stream): work-model
                 bytes of its launch (every neighbour-list fetch of the
                 reference's loop nest costs 16 + 4 d(v) bytes) / its average
-                launch time, against the 8 TB/s HBM peak;
+                launch time, against the 8 TB/s HBM peak; `traffic` is the
+                launch's measured HBM bytes (rocprofv3 FETCH_SIZE +
+                WRITE_SIZE, profiles/);
   cpu_baseline  the CPU oracle (clean-room OpenMP restatement of the
                 reference nests) timed on a seeded random sample of the same
                 roots, scaled to the full query;
@@ -58,6 +60,19 @@ def cpu_baseline(dag, roots_end, frac_den=1024, seed=7):
     return {"value": round(len(QUERIES) / est, 6), "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"{len(roots)} roots (uniform seeded 1/{frac_den} of [0,{roots_end})), P13+P14, "
                       f"{t_total:.1f}s measured, scaled x{frac_den}"}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "clique_p13_p14_rmat22_q0_traffic.json")
+
+
+def measured_traffic(kernel, scale):
+    """HBM bytes per launch (read + written) of `kernel` on the default
+    workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/ (scaled by the plain-copy calibration); None when not measured."""
+    if scale != SCALE or not os.path.exists(TRAFFIC_FILE):
+        return None
+    k = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(kernel)
+    return None if k is None else int(k["read_bytes"] + k["write_bytes"])
 
 
 def splitmix(seed, i):
@@ -277,9 +292,10 @@ def main():
         achieved = dbytes / (avg_ms * 1e-3) / 1e9
         total_bytes = {p: G.work_bytes(p, (0, q)) for p in QUERIES}
         G.free()
+    kname = f"{dname} ({dp})"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"{dname} ({dp})", "kernel_avg_ms": round(avg_ms, 3), "algorithmic_bytes": dbytes}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, args.scale),
+                "kernel": kname, "kernel_avg_ms": round(avg_ms, 3), "algorithmic_bytes": dbytes}
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
