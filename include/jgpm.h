@@ -12,7 +12,10 @@
  *                                  + NvsGraphGPU::init (include/graph_gpu_nvs.h:258-304)
  *   jgpm_count                     the round loop + name->kernel dispatch + count
  *                                  readback (src/jupiter_gpm.cu:356-908)
- *   jgpm_count_batch               no reference counterpart (one query per run there)
+ *   jgpm_count_batch, jgpm_batch_*,
+ *   jgpm_autotune_batch            no reference counterpart (one query per run there;
+ *                                  the reference's host chunk scheduler,
+ *                                  common/scheduler.cc, is never called)
  *   jgpm_serve_*                   no reference counterpart (continuous batching)
  *   jgpm_gen_* / jgpm_csr_*        host graph construction used by the CLI, tests
  *                                  and benchmarks; orientation follows
@@ -36,7 +39,7 @@ extern "C" {
 #define JGPM_E_IO           -5   /* file missing / short read / bad meta file */
 #define JGPM_E_BUDGET       -6   /* request exceeds the HBM budget of the serving loop */
 #define JGPM_E_GRAPHKIND    -7   /* pattern needs the symmetric CSR but the graph was uploaded as DAG-only */
-#define JGPM_E_BUSY         -8   /* serving queue full, retry */
+#define JGPM_E_BUSY         -8   /* serving queue full (retry), or a graph still used by a prepared batch */
 #define JGPM_E_NOTREADY     -9   /* serving result not ready yet */
 #define JGPM_E_STATE       -10   /* call not valid in the current state */
 #define JGPM_E_NODEVICE    -11   /* no HIP device available */
@@ -127,13 +130,24 @@ int  jgpm_last_profile(jgpm_ctx_t ctx, int n, const char** names, double* ms);
 int  jgpm_work_bytes_split(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
                            uint64_t* bytes, int n);
 
-/* Tuning knobs (autotuner output); 0 keeps the current value. */
+/* Tuning knobs (autotuner output).  In jgpm_set_tuning a field > 0 sets the
+ * knob (clamped to its range), 0 keeps the current value, < 0 restores the
+ * default. */
 typedef struct {
   int32_t small_max_deg;    /* roots with out-degree <= this use one wave each (<= 64)   */
   int32_t lds_max_deg;      /* roots up to this out-degree keep their bitmap in LDS      */
   int32_t gallop_ratio;     /* neighbour list longer than ratio*d(root) -> binary search  */
   int32_t waves_per_block;  /* waves per workgroup for the wave-per-root kernel           */
   int32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * num_cu              */
+  /* batched launches (jgpm_count_batch, jgpm_batch_*, serving) */
+  int32_t batch_hub_deg;    /* clique roots above this out-degree leave the wave-per-root
+                               persistent kernel for block-per-hub launches: 64 (one-word
+                               LDS rows) or 128 (two-word rows)                             */
+  int32_t batch_root_chunk; /* roots per work item of a clique query                      */
+  int32_t batch_edge_chunk; /* directed edges per work item of a P1..P12 query (P2 and P3,
+                               whose per-edge work is O(1) after the intersection, use 8x) */
+  int32_t batch_blocks_per_cu; /* persistent batched grid = this * num_cu, capped by the
+                                  kernel's occupancy                                       */
 } jgpm_tuning;
 int  jgpm_get_tuning(jgpm_ctx_t ctx, jgpm_tuning* t);
 int  jgpm_set_tuning(jgpm_ctx_t ctx, const jgpm_tuning* t);
@@ -148,6 +162,26 @@ typedef struct {
 } jgpm_query;
 int  jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* queries, int n, uint64_t* counts,
                       double* kernel_ms);
+
+/* Prepared batch: the query table, its buffers and every launch of one batched
+ * count (counter fills, the persistent kernel, the block-per-hub launches)
+ * captured once in a HIP graph and replayed by jgpm_batch_run.  The tuning in
+ * force at create time is baked in.  Queries must be TC or P1..P16.  A graph
+ * referenced by a live batch cannot be freed (JGPM_E_BUSY); free the batch
+ * first.  Runs on the context stream; one thread per context at a time. */
+typedef struct jgpm_batch_s* jgpm_batch_t;
+int  jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* queries, int n, jgpm_batch_t* out);
+int  jgpm_batch_run(jgpm_batch_t b, uint64_t* counts, double* kernel_ms);
+int  jgpm_batch_free(jgpm_batch_t b);
+
+/* Autotuner for batched launches: times the batch (device time, HIP events,
+ * best of `reps`) under candidate settings of batch_hub_deg, batch_root_chunk,
+ * batch_edge_chunk and batch_blocks_per_cu (coordinate descent from the current
+ * tuning), requires every candidate's counts to equal the first run's, and
+ * installs the fastest setting in the context.  best / best_ms may be NULL.
+ * Returns JGPM_E_STATE if two candidates disagree on a count. */
+int  jgpm_autotune_batch(jgpm_ctx_t ctx, const jgpm_query* queries, int n, int reps,
+                         jgpm_tuning* best, double* best_ms);
 
 /* ---- serving: continuous batching over device-resident graphs ---------- */
 typedef struct {

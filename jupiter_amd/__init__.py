@@ -57,7 +57,9 @@ _lib = None
 class _Tuning(ctypes.Structure):
     _fields_ = [("small_max_deg", ctypes.c_int32), ("lds_max_deg", ctypes.c_int32),
                 ("gallop_ratio", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
-                ("blocks_per_cu", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("batch_hub_deg", ctypes.c_int32),
+                ("batch_root_chunk", ctypes.c_int32), ("batch_edge_chunk", ctypes.c_int32),
+                ("batch_blocks_per_cu", ctypes.c_int32)]
 
 
 class _Query(ctypes.Structure):
@@ -102,6 +104,11 @@ def lib():
         "jgpm_set_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
         "jgpm_count_batch": ([P, ctypes.POINTER(_Query), ctypes.c_int, ctypes.POINTER(U64),
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_batch_create": ([P, ctypes.POINTER(_Query), ctypes.c_int, PP], ctypes.c_int),
+        "jgpm_batch_run": ([P, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_batch_free": ([P], ctypes.c_int),
+        "jgpm_autotune_batch": ([P, ctypes.POINTER(_Query), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Tuning),
+                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "jgpm_serve_start": ([P, ctypes.POINTER(_ServeCfg), PP], ctypes.c_int),
         "jgpm_serve_submit": ([P, P, ctypes.c_int, I64, I64, ctypes.POINTER(U64)], ctypes.c_int),
         "jgpm_serve_poll": ([P, U64, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -299,18 +306,64 @@ class Context:
         _check(lib().jgpm_graph_load(self._h, prefix.encode(), int(dag_only), ctypes.byref(g._h)), "graph_load")
         return g
 
-    def count_batch(self, queries):
-        """queries: list of (Graph, pattern, root_begin, root_end). Returns (counts, kernel_ms)."""
+    @staticmethod
+    def _queries(queries):
         arr = (_Query * len(queries))()
         for i, (g, p, rb, re) in enumerate(queries):
             arr[i].graph = g._h
             arr[i].pattern = pattern_id(p)
             arr[i].root_begin = rb
             arr[i].root_end = -1 if re is None else re
+        return arr
+
+    def count_batch(self, queries):
+        """queries: list of (Graph, pattern, root_begin, root_end). Returns (counts, kernel_ms)."""
+        arr = self._queries(queries)
         out = (ctypes.c_uint64 * len(queries))()
         ms = ctypes.c_double()
         _check(lib().jgpm_count_batch(self._h, arr, len(queries), out, ctypes.byref(ms)), "count_batch")
         return [int(x) for x in out], ms.value
+
+    def prepare_batch(self, queries):
+        """A Batch: the queries' launches captured once in a HIP graph, replayed by Batch.run()."""
+        return Batch(self, queries)
+
+    def autotune_batch(self, queries, reps=2):
+        """Tune the batched-launch knobs on these queries; installs and returns (tuning, best_ms)."""
+        arr = self._queries(queries)
+        t = _Tuning()
+        ms = ctypes.c_double()
+        _check(lib().jgpm_autotune_batch(self._h, arr, len(queries), reps, ctypes.byref(t), ctypes.byref(ms)),
+               "autotune_batch")
+        return {f: getattr(t, f) for f, _ in _Tuning._fields_}, ms.value
+
+
+class Batch:
+    """Prepared batched count (jgpm_batch_*): run() replays the captured launches."""
+
+    def __init__(self, ctx, queries):
+        self.ctx = ctx
+        self.n = len(queries)
+        self._h = ctypes.c_void_p()
+        arr = Context._queries(queries)
+        _check(lib().jgpm_batch_create(ctx._h, arr, self.n, ctypes.byref(self._h)), "batch_create")
+
+    def run(self):
+        out = (ctypes.c_uint64 * self.n)()
+        ms = ctypes.c_double()
+        _check(lib().jgpm_batch_run(self._h, out, ctypes.byref(ms)), "batch_run")
+        return [int(x) for x in out], ms.value
+
+    def free(self):
+        if self._h:
+            _check(lib().jgpm_batch_free(self._h), "batch_free")
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
 
 
 class Graph:
@@ -324,7 +377,7 @@ class Graph:
 
     def free(self):
         if self._h:
-            lib().jgpm_graph_free(self._h)
+            _check(lib().jgpm_graph_free(self._h), "graph_free")
             self._h = ctypes.c_void_p()
 
     def info(self):

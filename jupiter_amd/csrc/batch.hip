@@ -7,9 +7,11 @@
 // the persistent grid takes the next chunk from one device-wide atomic queue
 // (chunks of expensive queries first), processes it with the same device
 // routines as the single-query kernels, and adds the chunk's count to that
-// query's uint64 slot.  Clique roots with out-degree <= 128 run inside the
-// wave (one- or two-word bitmap rows in LDS); the few wider roots of a query
-// run afterwards in the block-per-root tier kernels, into the same slot.
+// query's uint64 slot.  Clique roots with out-degree up to the hub threshold
+// (64 or 128, a tuning knob) run inside the wave (one- or two-word bitmap rows
+// in LDS); the few wider roots of a query run afterwards in the block-per-root
+// tier kernels, into the same slot.  All launches of a batch can be captured
+// once in a HIP graph (prepared batches) and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +33,7 @@ struct BatchArgs {
   unsigned long long* qctr;
   int32_t* scr;           // per-wave list scratch of the symmetric patterns
   int64_t scr_stride;
+  int32_t hub;            // clique roots above this out-degree are left to the hub launches (<= 128)
 };
 
 constexpr int BWPB = 4;  // independent waves per block
@@ -105,7 +108,7 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
         const int64_t v0 = Q.base + u;
         const int64_t b0 = Q.rp[v0];
         const int d = (int)(Q.rp[v0 + 1] - b0);
-        if (d < Q.kind - 1 || d > 128) continue;
+        if (d < Q.kind - 1 || d > a.hub) continue;
         switch (Q.kind) {
           case 3: batch_root<3>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
           case 4: batch_root<4>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
@@ -152,6 +155,118 @@ static int batch_kind(int pattern) {
   return 0;
 }
 
+bool batch_supported(int pattern) { return batch_kind(pattern) != 0; }
+
+// Device buffers one batched launch reads and writes.
+struct BatchBufs {
+  jk::BQ* bq = nullptr;
+  int64_t* bcs = nullptr;
+  uint64_t* bcnt = nullptr;
+  unsigned* bq32 = nullptr;             // queue counters of the hub launches, NUM_TIERS per query
+  unsigned long long* qctr = nullptr;   // chunk queue of the persistent kernel
+  int32_t* sscr = nullptr;              // list scratch, grid * BWPB * NLIST * stride ints
+  int64_t sscr_stride = 1;
+  uint64_t* gscr = nullptr;             // global-row tier scratch (nullptr: the context's)
+  size_t gscr_words = 0;
+};
+
+__global__ void k_zero_u64(uint64_t* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_zero_u32(unsigned* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+static int batch_grid(jgpm_ctx_s* c) {
+  if (c->batch_occ == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, jk::batch_kernel, 64 * jk::BWPB, 0) != hipSuccess || nb < 1)
+      nb = 2;
+    c->batch_occ = nb;
+  }
+  const int want = c->tune.batch_blocks_per_cu > 0 ? c->tune.batch_blocks_per_cu : c->batch_occ;
+  return std::min(want, c->batch_occ) * c->num_cu;
+}
+
+// Host side of a batch: the query table (work items sized by the tuning),
+// its chunk prefix and the list-scratch stride.  May build a graph's lazy
+// auxiliary arrays on st (kernels and allocations), so it is never captured.
+static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>& hq, std::vector<int64_t>& cs,
+                      int32_t& max_sym, hipStream_t st) {
+  const int rchunk = std::max(1, c->tune.batch_root_chunk);
+  const int echunk = std::max(1, c->tune.batch_edge_chunk);
+  hq.assign(n, jk::BQ{});
+  max_sym = 1;
+  for (int i = 0; i < n; ++i) {
+    jgpm_graph_s* g = q[i].graph;
+    const int kind = batch_kind(q[i].pattern);
+    int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;
+    jk::BQ& b = hq[i];
+    b.kind = kind;
+    if (kind < 100) {
+      b.rp = g->dag.rp;
+      b.ci = g->dag.ci;
+      b.ord = g->d_ord;
+      b.base = rb;
+      b.ntasks = re - rb;
+      b.chunk = rchunk;
+    } else {
+      const int p = kind - 100;
+      int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
+      if (rc) return rc;
+      b.rp = g->sym.rp;
+      b.ci = g->sym.ci;
+      b.src = g->d_src;
+      b.tri = g->d_tri;
+      b.nsum = g->d_nsum;
+      b.base = g->sym_rp_h[rb];
+      b.ntasks = g->sym_rp_h[re] - b.base;
+      b.chunk = (p == 2 || p == 3) ? 8 * echunk : echunk;
+      max_sym = std::max(max_sym, g->sym_mmin);
+    }
+  }
+  cs.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) cs[i + 1] = cs[i] + (hq[i].ntasks + hq[i].chunk - 1) / hq[i].chunk;
+  return JGPM_OK;
+}
+
+// Every launch of one batch, and nothing else (no allocation, host copy or
+// memset), so that the sequence can be captured: counter fills, the
+// persistent kernel, then the hub launches of each clique query into its slot.
+static int launch_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, const std::vector<jk::BQ>& hq, const BatchBufs& b,
+                        int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, st, b.bcnt, n);
+  hipLaunchKernelGGL(k_zero_u32, dim3((jk::NUM_TIERS * n + 255) / 256), dim3(256), 0, st, b.bq32, jk::NUM_TIERS * n);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(b.qctr), 1);
+  const int hub = c->tune.batch_hub_deg <= 64 ? 64 : 128;
+  jk::BatchArgs a{};
+  a.q = b.bq;
+  a.cstart = b.bcs;
+  a.nq = n;
+  a.ratio = std::max(1, c->tune.gallop_ratio);
+  a.counts = b.bcnt;
+  a.qctr = b.qctr;
+  a.scr = b.sscr;
+  a.scr_stride = b.sscr_stride;
+  a.hub = hub;
+  prof_begin(c, st, "batch_persistent");
+  hipLaunchKernelGGL(jk::batch_kernel, dim3(grid), dim3(64 * jk::BWPB), 0, st, a);
+  prof_end(c, st);
+  if (hipGetLastError() != hipSuccess) return JGPM_E_HIP;
+  const int first_tier = hub <= 64 ? jk::TIER_LDS2 : jk::TIER_LDS4;
+  for (int i = 0; i < n; ++i) {
+    if (hq[i].kind >= 100) continue;
+    jgpm_graph_s* g = q[i].graph;
+    int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;
+    int rc = enqueue_clique(g, hq[i].kind, false, rb, re, 0, st, false, first_tier, b.bcnt + i,
+                            b.bq32 + (size_t)jk::NUM_TIERS * i, b.gscr, b.gscr_words);
+    if (rc) return rc;
+  }
+  return JGPM_OK;
+}
+
 static int ensure_batch_buffers(jgpm_ctx_s* c, int n) {
   if (c->bcap >= n) return JGPM_OK;
   if (hipStreamSynchronize(c->st) != hipSuccess) return JGPM_E_HIP;
@@ -174,62 +289,18 @@ static int ensure_batch_buffers(jgpm_ctx_s* c, int n) {
   return JGPM_OK;
 }
 
-__global__ void k_zero_u64(uint64_t* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-__global__ void k_zero_u32(unsigned* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-// Enqueue one persistent launch for queries q[0..n) (all kinds supported
-// by the kernel), plus the wide-root tier launches.  Counts land in
-// c->d_bcnt[i].  Host vectors must stay alive until the stream is synced.
+// Eager batched launch on the context's buffers for queries q[0..n) (all of
+// kinds the kernel runs).  Counts land in c->d_bcnt[i].  hq / cs must stay
+// alive until the stream is synchronised.
 int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>& hq, std::vector<int64_t>& cs,
                   hipStream_t st) {
   int rc = ensure_batch_buffers(c, n);
   if (rc) return rc;
-  hq.assign(n, jk::BQ{});
-  int32_t max_sym_deg = 1;
-  for (int i = 0; i < n; ++i) {
-    jgpm_graph_s* g = q[i].graph;
-    const int kind = batch_kind(q[i].pattern);
-    int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;
-    jk::BQ& b = hq[i];
-    b.kind = kind;
-    if (kind < 100) {
-      b.rp = g->dag.rp;
-      b.ci = g->dag.ci;
-      b.ord = g->d_ord;
-      b.base = rb;
-      b.ntasks = re - rb;
-      b.chunk = 16;
-    } else {
-      const int p = kind - 100;
-      rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
-      if (rc) return rc;
-      b.rp = g->sym.rp;
-      b.ci = g->sym.ci;
-      b.src = g->d_src;
-      b.tri = g->d_tri;
-      b.nsum = g->d_nsum;
-      b.base = g->sym_rp_h[rb];
-      b.ntasks = g->sym_rp_h[re] - b.base;
-      b.chunk = (p == 2 || p == 3) ? 64 : 8;
-      max_sym_deg = std::max(max_sym_deg, g->sym_mmin);
-    }
-  }
-  cs.assign(n + 1, 0);
-  for (int i = 0; i < n; ++i) cs[i + 1] = cs[i] + (hq[i].ntasks + hq[i].chunk - 1) / hq[i].chunk;
-  if (c->batch_blocks_per_cu == 0) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, jk::batch_kernel, 64 * jk::BWPB, 0) != hipSuccess || nb < 1)
-      nb = 2;
-    c->batch_blocks_per_cu = nb;
-  }
-  const int grid = std::min(std::max(1, c->tune.blocks_per_cu), c->batch_blocks_per_cu) * c->num_cu;
-  const size_t need = (size_t)grid * jk::BWPB * jk::symd::NLIST * (size_t)max_sym_deg;
+  int32_t max_sym = 1;
+  rc = plan_batch(c, q, n, hq, cs, max_sym, st);
+  if (rc) return rc;
+  const int grid = batch_grid(c);
+  const size_t need = (size_t)grid * jk::BWPB * jk::symd::NLIST * (size_t)max_sym;
   if (c->sscr_ints < need) {
     if (hipStreamSynchronize(st) != hipSuccess) return JGPM_E_HIP;
     if (c->d_sscr) (void)hipFree(c->d_sscr);
@@ -241,46 +312,25 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
   if (hipMemcpyAsync(c->d_bq, hq.data(), sizeof(jk::BQ) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(c->d_bcs, cs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess)
     return JGPM_E_HIP;
-  hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, st, c->d_bcnt, n);
-  hipLaunchKernelGGL(k_zero_u32, dim3((jk::NUM_TIERS * n + 255) / 256), dim3(256), 0, st, c->d_bq32,
-                     jk::NUM_TIERS * n);
-  jk::BatchArgs a{};
-  a.q = (const jk::BQ*)c->d_bq;
-  a.cstart = c->d_bcs;
-  a.nq = n;
-  a.ratio = std::max(1, c->tune.gallop_ratio);
-  a.counts = c->d_bcnt;
-  a.qctr = c->d_q64 + (CTX_SLOTS - 2);
-  a.scr = c->d_sscr;
-  a.scr_stride = max_sym_deg;
-  prof_begin(c, st, "batch_persistent");
-  hipLaunchKernelGGL(jk::batch_kernel, dim3(grid), dim3(64 * jk::BWPB), 0, st, a);
-  prof_end(c, st);
-  if (hipGetLastError() != hipSuccess) return JGPM_E_HIP;
-  // wide clique roots (out-degree > 128) of each query, into its slot
-  for (int i = 0; i < n; ++i) {
-    if (hq[i].kind >= 100) continue;
-    jgpm_graph_s* g = q[i].graph;
-    int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;
-    rc = enqueue_clique(g, hq[i].kind, false, rb, re, 0, st, false, jk::TIER_LDS4, c->d_bcnt + i,
-                        c->d_bq32 + (size_t)jk::NUM_TIERS * i);
-    if (rc) return rc;
-  }
-  return JGPM_OK;
+  BatchBufs b;
+  b.bq = static_cast<jk::BQ*>(c->d_bq);
+  b.bcs = c->d_bcs;
+  b.bcnt = c->d_bcnt;
+  b.bq32 = c->d_bq32;
+  b.qctr = c->d_q64 + (CTX_SLOTS - 2);
+  b.sscr = c->d_sscr;
+  b.sscr_stride = max_sym;
+  return launch_batch(c, q, n, hq, b, grid, st);
 }
-
-bool batch_supported(int pattern) { return batch_kind(pattern) != 0; }
 
 int enqueue_batch_q(jgpm_ctx_s* c, const jgpm_query* q, int n, hipStream_t st) {
   // the staging lives on the context until the next batched launch
   return enqueue_batch(c, q, n, c->bq_host, c->bcs_host, st);
 }
 
-}  // namespace jgpm
-
-extern "C" int jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, uint64_t* counts, double* kernel_ms) {
-  using namespace jgpm;
-  if (!ctx || n < 0 || (n > 0 && (!q || !counts))) return JGPM_E_INVALID;
+// Checks shared by the batched entry points; `kernel_only` rejects patterns
+// the persistent kernel does not run.
+static int check_queries(jgpm_ctx_s* ctx, const jgpm_query* q, int n, bool kernel_only) {
   for (int i = 0; i < n; ++i) {
     if (!q[i].graph || q[i].graph->ctx != ctx) return JGPM_E_INVALID;
     if (q[i].pattern < 0 || q[i].pattern >= JGPM_NUM_PATTERNS) return JGPM_E_PATTERN;
@@ -288,6 +338,7 @@ extern "C" int jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, uint
     const int64_t re = q[i].root_end < 0 ? nv : q[i].root_end;
     if (q[i].root_begin < 0 || q[i].root_begin > re || re > nv) return JGPM_E_INVALID;
     const int kind = batch_kind(q[i].pattern);
+    if (kernel_only && !kind) return JGPM_E_PATTERN;
     if (kind >= 100 && !q[i].graph->has_sym) return JGPM_E_GRAPHKIND;
     if (kind && kind < 100 && !q[i].graph->huge_h.empty()) {
       auto& hv = q[i].graph->huge_h;
@@ -295,19 +346,35 @@ extern "C" int jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, uint
       if (lo != hv.end() && *lo < re) return JGPM_E_BUDGET;
     }
   }
-  if (n == 0) {
-    if (kernel_ms) *kernel_ms = 0;
-    return JGPM_OK;
-  }
-  // kernel-supported queries, most expensive first
-  std::vector<int> idx, rest;
-  for (int i = 0; i < n; ++i) (batch_kind(q[i].pattern) ? idx : rest).push_back(i);
+  return JGPM_OK;
+}
+
+// Kernel-run queries in launch order: most expensive first.
+static std::vector<int> cost_order(const jgpm_query* q, const std::vector<int>& idx_in) {
+  std::vector<int> idx = idx_in;
   auto cost = [&](int i) {
     const jgpm_graph_s* g = q[i].graph;
     const int kind = batch_kind(q[i].pattern);
     return (double)(kind < 100 ? g->dag.nnz : g->sym.nnz) * (kind < 100 ? kind : 8);
   };
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  return idx;
+}
+
+}  // namespace jgpm
+
+extern "C" int jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, uint64_t* counts, double* kernel_ms) {
+  using namespace jgpm;
+  if (!ctx || n < 0 || (n > 0 && (!q || !counts))) return JGPM_E_INVALID;
+  int st = check_queries(ctx, q, n, false);
+  if (st) return st;
+  if (n == 0) {
+    if (kernel_ms) *kernel_ms = 0;
+    return JGPM_OK;
+  }
+  std::vector<int> all, rest;
+  for (int i = 0; i < n; ++i) (batch_kind(q[i].pattern) ? all : rest).push_back(i);
+  const std::vector<int> idx = cost_order(q, all);
   std::vector<jgpm_query> qs;
   for (int i : idx) qs.push_back(q[i]);
   double total = 0;
@@ -315,8 +382,6 @@ extern "C" int jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, uint
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (hipSetDevice(ctx->dev) != hipSuccess) return JGPM_E_HIP;
     if (hipEventRecord(ctx->e0, ctx->st) != hipSuccess) return JGPM_E_HIP;
-    int st = ctx_reset_counters(ctx, ctx->st);
-    if (st) return st;
     ctx->nprof = 0;
     std::vector<jk::BQ> hq;
     std::vector<int64_t> cs;
@@ -341,10 +406,199 @@ extern "C" int jgpm_count_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, uint
   // composite patterns (induced motifs, vertex sums) run one by one
   for (int i : rest) {
     double m = 0;
-    int st = jgpm_count(q[i].graph, q[i].pattern, q[i].root_begin, q[i].root_end, &counts[i], &m);
+    st = jgpm_count(q[i].graph, q[i].pattern, q[i].root_begin, q[i].root_end, &counts[i], &m);
     if (st) return st;
     total += m;
   }
   if (kernel_ms) *kernel_ms = total;
+  return JGPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Prepared batches: own buffers, one captured HIP graph, replay per run.
+
+struct jgpm_batch_s {
+  jgpm_ctx_s* ctx = nullptr;
+  std::vector<jgpm_query> q;   // launch order
+  std::vector<int> idx;        // launch slot -> caller index
+  std::vector<jk::BQ> hq;
+  std::vector<int64_t> cs;
+  jgpm::BatchBufs b;
+  int grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> h;
+};
+
+static void batch_release(jgpm_batch_s* B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  void* bufs[] = {B->b.bq, B->b.bcs, B->b.bcnt, B->b.bq32, B->b.qctr, B->b.sscr, B->b.gscr};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  for (const jgpm_query& x : B->q) x.graph->nbatch--;
+  delete B;
+}
+
+extern "C" int jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* q, int n, jgpm_batch_t* out) {
+  using namespace jgpm;
+  if (!ctx || !out || n <= 0 || !q) return JGPM_E_INVALID;
+  int st = check_queries(ctx, q, n, true);
+  if (st) return st;
+  std::vector<int> all(n);
+  for (int i = 0; i < n; ++i) all[i] = i;
+  auto* B = new (std::nothrow) jgpm_batch_s();
+  if (!B) return JGPM_E_NOMEM;
+  B->ctx = ctx;
+  B->idx = cost_order(q, all);
+  for (int i : B->idx) {
+    B->q.push_back(q[i]);
+    q[i].graph->nbatch++;
+  }
+  B->h.assign(n, 0);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(ctx->st);
+    batch_release(B);
+    return code;
+  };
+  if (hipSetDevice(ctx->dev) != hipSuccess) return fail(JGPM_E_HIP);
+  int32_t max_sym = 1;
+  st = plan_batch(ctx, B->q.data(), n, B->hq, B->cs, max_sym, ctx->st);
+  if (st) return fail(st);
+  B->grid = batch_grid(ctx);
+  // global-row scratch only when a clique query has roots in that tier
+  bool glb = false;
+  for (int i = 0; i < n; ++i) {
+    if (B->hq[i].kind >= 100) continue;
+    const auto& v = B->q[i].graph->tier_h[jk::TIER_GLB32];
+    const int64_t rb = B->q[i].root_begin, re = B->q[i].root_end < 0 ? B->q[i].graph->dag.nv : B->q[i].root_end;
+    auto lo = std::lower_bound(v.begin(), v.end(), (int32_t)rb);
+    glb = glb || (lo != v.end() && *lo < re);
+  }
+  B->b.sscr_stride = max_sym;
+  const size_t sscr = (size_t)B->grid * jk::BWPB * jk::symd::NLIST * (size_t)max_sym;
+  if (glb) B->b.gscr_words = jk::tier_scratch_words(jk::TIER_GLB32) * (size_t)glb_tier_grid(ctx);
+  if (hipMalloc(&B->b.bq, sizeof(jk::BQ) * n) != hipSuccess ||
+      hipMalloc(&B->b.bcs, sizeof(int64_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&B->b.bcnt, sizeof(uint64_t) * n) != hipSuccess ||
+      hipMalloc(&B->b.bq32, sizeof(unsigned) * jk::NUM_TIERS * n) != hipSuccess ||
+      hipMalloc(&B->b.qctr, 64) != hipSuccess || hipMalloc(&B->b.sscr, sscr * sizeof(int32_t)) != hipSuccess ||
+      (glb && hipMalloc(&B->b.gscr, B->b.gscr_words * sizeof(uint64_t)) != hipSuccess))
+    return fail(JGPM_E_NOMEM);
+  if (hipMemcpyAsync(B->b.bq, B->hq.data(), sizeof(jk::BQ) * n, hipMemcpyHostToDevice, ctx->st) != hipSuccess ||
+      hipMemcpyAsync(B->b.bcs, B->cs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, ctx->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(ctx->st) != hipSuccess)
+    return fail(JGPM_E_HIP);
+  // capture every launch of the batch (no allocations, host copies or
+  // memsets happen inside launch_batch; profiling events stay out)
+  const bool prof = ctx->prof_on;
+  ctx->prof_on = false;
+  if (hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    ctx->prof_on = prof;
+    return fail(JGPM_E_HIP);
+  }
+  st = launch_batch(ctx, B->q.data(), n, B->hq, B->b, B->grid, ctx->st);
+  const hipError_t ce = hipStreamEndCapture(ctx->st, &B->graph);
+  ctx->prof_on = prof;
+  if (st) return fail(st);
+  if (ce != hipSuccess || !B->graph) return fail(JGPM_E_HIP);
+  if (hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) return fail(JGPM_E_HIP);
+  *out = B;
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_batch_run(jgpm_batch_t B, uint64_t* counts, double* kernel_ms) {
+  if (!B || !counts) return JGPM_E_INVALID;
+  jgpm_ctx_s* ctx = B->ctx;
+  const int n = (int)B->q.size();
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->dev) != hipSuccess) return JGPM_E_HIP;
+  ctx->nprof = 0;
+  if (hipEventRecord(ctx->e0, ctx->st) != hipSuccess || hipGraphLaunch(B->exec, ctx->st) != hipSuccess ||
+      hipEventRecord(ctx->e1, ctx->st) != hipSuccess ||
+      hipMemcpyAsync(B->h.data(), B->b.bcnt, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, ctx->st) != hipSuccess ||
+      hipStreamSynchronize(ctx->st) != hipSuccess)
+    return JGPM_E_HIP;
+  for (int k = 0; k < n; ++k) counts[B->idx[k]] = B->h[k];
+  if (kernel_ms) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->e0, ctx->e1);
+    *kernel_ms = ms;
+  }
+  return JGPM_OK;
+}
+
+extern "C" int jgpm_batch_free(jgpm_batch_t B) {
+  if (!B) return JGPM_E_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(B->ctx->mu);
+    (void)hipSetDevice(B->ctx->dev);
+    (void)hipStreamSynchronize(B->ctx->st);
+  }
+  batch_release(B);
+  return JGPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: coordinate descent over the batched-launch knobs.
+
+extern "C" int jgpm_autotune_batch(jgpm_ctx_t ctx, const jgpm_query* q, int n, int reps, jgpm_tuning* best,
+                                   double* best_ms) {
+  using namespace jgpm;
+  if (!ctx || n <= 0 || !q) return JGPM_E_INVALID;
+  int st = check_queries(ctx, q, n, false);
+  if (st) return st;
+  reps = std::max(1, std::min(reps, 16));
+  const jgpm_tuning start = ctx->tune;
+  std::vector<uint64_t> ref(n), got(n);
+  bool have_ref = false;
+  auto eval = [&](const jgpm_tuning& t, double& ms) -> int {
+    ctx->tune = t;
+    ms = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      double m = 0;
+      int rc = jgpm_count_batch(ctx, q, n, got.data(), &m);
+      if (rc) return rc;
+      ms = std::min(ms, m);
+      if (!have_ref) {
+        ref = got;
+        have_ref = true;
+      } else if (got != ref) {
+        return JGPM_E_STATE;  // every setting must give the same exact counts
+      }
+    }
+    return JGPM_OK;
+  };
+  double cur_ms = 0, m = 0;
+  st = eval(start, m);  // warm-up: builds lazy per-graph arrays
+  if (!st) st = eval(start, cur_ms);
+  jgpm_tuning cur = start;
+  const int occ = std::max(1, ctx->batch_occ);
+  const int hub_opt[] = {64, 128};
+  const int rchunk_opt[] = {4, 16, 64};
+  const int echunk_opt[] = {2, 8, 32};
+  const int bpc_opt[] = {occ, std::max(1, occ / 2)};
+  for (int knob = 0; knob < 4 && !st; ++knob) {
+    const int* opt = knob == 0 ? hub_opt : knob == 1 ? rchunk_opt : knob == 2 ? echunk_opt : bpc_opt;
+    const int nopt = knob == 0 || knob == 3 ? 2 : 3;
+    for (int k = 0; k < nopt && !st; ++k) {
+      jgpm_tuning t = cur;
+      int32_t* f = knob == 0 ? &t.batch_hub_deg : knob == 1 ? &t.batch_root_chunk
+                 : knob == 2 ? &t.batch_edge_chunk : &t.batch_blocks_per_cu;
+      if (*f == opt[k] || (knob == 3 && *f <= 0 && opt[k] == occ)) continue;
+      *f = opt[k];
+      st = eval(t, m);
+      if (!st && m < cur_ms) {
+        cur = t;
+        cur_ms = m;
+      }
+    }
+  }
+  ctx->tune = st ? start : cur;
+  if (st) return st;
+  if (best) *best = cur;
+  if (best_ms) *best_ms = cur_ms;
   return JGPM_OK;
 }

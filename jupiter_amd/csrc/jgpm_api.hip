@@ -211,6 +211,20 @@ static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_
 }  // namespace jgpm
 
 // ---------------------------------------------------------------------------
+static jgpm_tuning default_tuning() {
+  jgpm_tuning t{};
+  t.small_max_deg = 64;
+  t.lds_max_deg = 512;
+  t.gallop_ratio = 32;
+  t.waves_per_block = 4;
+  t.blocks_per_cu = 8;
+  t.batch_hub_deg = 128;
+  t.batch_root_chunk = 16;
+  t.batch_edge_chunk = 8;
+  t.batch_blocks_per_cu = 0;  // 0: the batched kernel's occupancy
+  return t;
+}
+
 extern "C" int jgpm_init(int dev, jgpm_ctx_t* out) {
   if (!out) return JGPM_E_INVALID;
   int n = 0;
@@ -242,11 +256,7 @@ extern "C" int jgpm_init(int dev, jgpm_ctx_t* out) {
     delete c;
     return JGPM_E_NOMEM;
   }
-  c->tune.small_max_deg = 64;
-  c->tune.lds_max_deg = 512;
-  c->tune.gallop_ratio = 32;
-  c->tune.waves_per_block = 4;
-  c->tune.blocks_per_cu = 8;
+  c->tune = default_tuning();
   *out = c;
   return JGPM_OK;
 }
@@ -290,11 +300,22 @@ extern "C" int jgpm_get_tuning(jgpm_ctx_t c, jgpm_tuning* t) {
 
 extern "C" int jgpm_set_tuning(jgpm_ctx_t c, const jgpm_tuning* t) {
   if (!c || !t) return JGPM_E_INVALID;
-  if (t->gallop_ratio > 0) c->tune.gallop_ratio = t->gallop_ratio;
-  if (t->blocks_per_cu > 0) c->tune.blocks_per_cu = std::min(t->blocks_per_cu, 64);
-  if (t->small_max_deg > 0) c->tune.small_max_deg = std::min(t->small_max_deg, 64);
-  if (t->lds_max_deg > 0) c->tune.lds_max_deg = t->lds_max_deg;
-  if (t->waves_per_block > 0) c->tune.waves_per_block = t->waves_per_block;
+  // per field: > 0 sets (clamped), 0 keeps, < 0 restores the default
+  const jgpm_tuning d = default_tuning();
+  auto put = [](int32_t v, int32_t def, int32_t& dst, int32_t lo, int32_t hi) {
+    if (v < 0) dst = def;
+    else if (v > 0) dst = std::min(std::max(v, lo), hi);
+  };
+  put(t->small_max_deg, d.small_max_deg, c->tune.small_max_deg, 1, 64);
+  put(t->lds_max_deg, d.lds_max_deg, c->tune.lds_max_deg, 1, 1 << 20);
+  put(t->gallop_ratio, d.gallop_ratio, c->tune.gallop_ratio, 1, 1 << 20);
+  put(t->waves_per_block, d.waves_per_block, c->tune.waves_per_block, 1, 16);
+  put(t->blocks_per_cu, d.blocks_per_cu, c->tune.blocks_per_cu, 1, 64);
+  put(t->batch_hub_deg, d.batch_hub_deg, c->tune.batch_hub_deg, 64, 128);
+  if (c->tune.batch_hub_deg != 64) c->tune.batch_hub_deg = 128;  // the two LDS row tiles the kernel has
+  put(t->batch_root_chunk, d.batch_root_chunk, c->tune.batch_root_chunk, 1, 4096);
+  put(t->batch_edge_chunk, d.batch_edge_chunk, c->tune.batch_edge_chunk, 1, 4096);
+  put(t->batch_blocks_per_cu, d.batch_blocks_per_cu, c->tune.batch_blocks_per_cu, 1, 64);
   return JGPM_OK;
 }
 
@@ -346,6 +367,7 @@ extern "C" int jgpm_graph_load(jgpm_ctx_t c, const char* prefix, int dag_only, j
 
 extern "C" int jgpm_graph_free(jgpm_graph_t g) {
   if (!g) return JGPM_E_INVALID;
+  if (g->nbatch > 0) return JGPM_E_BUSY;  // a prepared batch still launches on it
   (void)hipSetDevice(g->ctx->dev);
   (void)hipStreamSynchronize(g->ctx->st);
   free_csr(g->sym);
@@ -397,8 +419,11 @@ int clique_k(int pattern) {
 static const char* const kTierName[jk::NUM_TIERS] = {"clique_wave64", "clique_lds128", "clique_lds256",
                                                      "clique_lds512", "clique_lds1024", "clique_glb2048"};
 
+int glb_tier_grid(const jgpm_ctx_s* c) { return 4 * c->num_cu; }
+
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
-                   bool split_slots, int first_tier, uint64_t* out_ptr, unsigned* qctr_base) {
+                   bool split_slots, int first_tier, uint64_t* out_ptr, unsigned* qctr_base, uint64_t* gscr,
+                   size_t gscr_words) {
   jgpm_ctx_s* c = g->ctx;
   if (!g->huge_h.empty()) {
     auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
@@ -438,10 +463,16 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
       grid = (int)std::min<int64_t>(maxgrid, n);
       size_t sw = jk::tier_scratch_words(tier);
       if (sw) {
-        grid = (int)std::min<int64_t>(grid, 4 * (int64_t)c->num_cu);
-        int s2 = ensure_scratch(c, sw * (size_t)grid);
-        if (s2) return s2;
-        a.gscr = c->d_scr;
+        grid = (int)std::min<int64_t>(grid, glb_tier_grid(c));
+        if (gscr) {  // caller-owned scratch (prepared batches): no allocation here
+          grid = (int)std::min<int64_t>(grid, (int64_t)(gscr_words / sw));
+          if (grid < 1) return JGPM_E_NOMEM;
+          a.gscr = gscr;
+        } else {
+          int s2 = ensure_scratch(c, sw * (size_t)grid);
+          if (s2) return s2;
+          a.gscr = c->d_scr;
+        }
       }
     }
     prof_begin(c, st, kTierName[tier]);

@@ -39,7 +39,7 @@ struct jgpm_ctx_s {
   uint64_t* d_bcnt = nullptr;
   unsigned* d_bq32 = nullptr;  // queue counters of the follow-up tier launches
   int bcap = 0;
-  int batch_blocks_per_cu = 0;  // resident blocks of the batched kernel
+  int batch_occ = 0;  // resident blocks per CU of the batched kernel (occupancy)
   std::vector<jk::BQ> bq_host; // staging of the last batched launch
   std::vector<int64_t> bcs_host;
   uint64_t* d_scr = nullptr;  // row scratch of the global-row tier
@@ -69,6 +69,7 @@ struct jgpm_graph_s {
   int32_t* d_tri = nullptr;                    // triangles per symmetric edge (lazy)
   int64_t* d_nsum = nullptr;                   // neighbour-degree sums (lazy)
   int64_t bytes = 0;
+  int nbatch = 0;                              // prepared batches that reference this graph
 };
 
 namespace jgpm {
@@ -77,7 +78,9 @@ int ensure_scratch(jgpm_ctx_s* c, size_t words);
 int clique_k(int pattern);
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
                    bool split_slots = false, int first_tier = 0, uint64_t* out_ptr = nullptr,
-                   unsigned* qctr_base = nullptr);
+                   unsigned* qctr_base = nullptr, uint64_t* gscr = nullptr, size_t gscr_words = 0);
+// Blocks of the global-row tier one launch may use, and their row scratch.
+int glb_tier_grid(const jgpm_ctx_s* c);
 int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st);
 // Batched persistent launch over queries the kernel supports (TC, P1..P16);
 // counts land in c->d_bcnt[0..n).  hq / cs are host staging that must stay

@@ -196,9 +196,10 @@ extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, in
     auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
     if (lo != g->huge_h.end() && *lo < re) return JGPM_E_BUDGET;
   }
-  const int grid = std::max(1, std::min(s->ctx->tune.blocks_per_cu, s->ctx->batch_blocks_per_cu > 0
-                                                                      ? s->ctx->batch_blocks_per_cu : 3)) *
-                   s->ctx->num_cu;
+  // grid of the batched kernel (its occupancy is known after the first launch)
+  const int occ = s->ctx->batch_occ > 0 ? s->ctx->batch_occ : 3;
+  const int want = s->ctx->tune.batch_blocks_per_cu > 0 ? s->ctx->tune.batch_blocks_per_cu : occ;
+  const int grid = std::max(1, std::min(want, occ)) * s->ctx->num_cu;
   Req r;
   r.g = g;
   r.pattern = pattern;
