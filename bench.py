@@ -83,51 +83,83 @@ def splitmix(seed, i):
 
 def workload_batch(args):
     """Many independent graphs, each paired with one pattern from
-    {TC, P13, P1, P2}: one persistent batched launch vs one plain launch per
-    query.  Full size: --batch-graphs 2048 --batch-max-n 262144."""
+    {TC, P13, P1, P2}: the batch autotuned, captured once in a HIP graph and
+    replayed, vs one plain launch per query.  The default is the full
+    configuration (2,048 graphs, n up to 262,144)."""
     rng = np.random.default_rng(SEED)
     specs = []
     t0 = time.time()
-    csrs = []
+    edges = 0
+    ctx = J.Context(0)
+    Gs = []
     for i in range(args.batch_graphs):
         n = int(np.exp(rng.uniform(np.log(1000), np.log(args.batch_max_n))))
         sd = splitmix(SEED, i)
         if i % 2 == 0:
             md = float(rng.uniform(8, 64))
-            csrs.append(J.erdos_renyi(n, md, seed=sd))
+            c = J.erdos_renyi(n, md, seed=sd)
             kind = f"ER(n={n},deg={md:.1f})"
         else:
             m = int(rng.integers(4, 17))
-            csrs.append(J.barabasi_albert(n, m, seed=sd))
+            c = J.barabasi_albert(n, m, seed=sd)
             kind = f"BA(n={n},m={m})"
         specs.append((kind, ["TC", "P13", "P1", "P2"][int(rng.integers(0, 4))]))
-    edges = sum(c.nnz // 2 for c in csrs)
-    log(f"generated {len(csrs)} graphs, {edges} edges ({time.time() - t0:.1f}s)")
-    with J.Context(0) as ctx:
-        Gs = [ctx.upload(c) for c in csrs]
+        edges += c.nnz // 2
+        Gs.append(ctx.upload(c))  # resident; the host copy is dropped
+        if (i + 1) % 128 == 0:
+            log(f"generated and uploaded {i + 1} graphs, {edges} edges ({time.time() - t0:.1f}s)")
+    log(f"generated {len(Gs)} graphs, {edges} edges ({time.time() - t0:.1f}s)")
+    with ctx:
         queries = [(G, p, 0, None) for G, (_, p) in zip(Gs, specs)]
         for _ in range(args.warmup):
             ctx.count_batch(queries)
+        # eager batched launch with the default knobs
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            batch_counts, batch_ms = ctx.count_batch(queries)
-        t_batch = (time.perf_counter() - t0) / args.steps
+            eager_counts, _ = ctx.count_batch(queries)
+        t_eager = (time.perf_counter() - t0) / args.steps
+        # autotune the knobs on this batch (not timed), then capture the batch
+        # in a HIP graph and time its replays
         t0 = time.perf_counter()
-        plain_counts = [G.count(p)[0] for G, p, _, _ in queries]
+        tuned, tuned_ms = ctx.autotune_batch(queries, reps=2)
+        t_tune = time.perf_counter() - t0
+        log(f"autotuned {tuned} ({tuned_ms:.2f} ms, {t_tune:.1f} s)")
+        with ctx.prepare_batch(queries) as B:
+            for _ in range(args.warmup):
+                B.run()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                batch_counts, batch_ms = B.run()
+            t_batch = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        plain_counts = []
+        plain_ms = {}
+        for k, (G, p, _, _) in enumerate(queries):
+            c, kms = G.count(p)
+            plain_counts.append(c)
+            plain_ms[p] = plain_ms.get(p, 0.0) + kms
+            if (k + 1) % 256 == 0:
+                log(f"plain {k + 1}/{len(queries)} ({time.perf_counter() - t0:.1f}s)")
         t_plain = time.perf_counter() - t0
-        if batch_counts != plain_counts:
-            bad = [i for i, (x, y) in enumerate(zip(batch_counts, plain_counts)) if x != y]
-            raise AssertionError(f"batched counts differ from plain counts at {bad[:10]}")
+        for name, got in (("batched", batch_counts), ("eager batched", eager_counts)):
+            if got != plain_counts:
+                bad = [i for i, (x, y) in enumerate(zip(got, plain_counts)) if x != y]
+                raise AssertionError(f"{name} counts differ from plain counts at {bad[:10]}")
         for G in Gs:
             G.free()
     nq = len(queries)
+    knobs = {k: tuned[k] for k in ("batch_hub_deg", "batch_root_chunk", "batch_edge_chunk", "batch_blocks_per_cu")}
     return {"metric": METRIC, "value": round(nq / t_batch, 3), "unit": "queries/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
             "config": {"workload": f"batched: {nq} graphs (half ER mean degree U[8,64], half BA m U{{4..16}}, "
                                    f"n log-uniform [1k,{args.batch_max_n}]), pattern uniform from "
-                                   "{TC,P13,P1,P2}, one persistent launch per step",
-                       "edges": edges, "plain_queries_per_s": round(nq / t_plain, 3),
+                                   "{TC,P13,P1,P2}; autotuned knobs, all launches of a step replayed from one "
+                                   "captured HIP graph",
+                       "edges": edges, "autotuned": knobs, "autotune_s": round(t_tune, 2),
+                       "eager_batched_queries_per_s": round(nq / t_eager, 3),
+                       "plain_queries_per_s": round(nq / t_plain, 3),
+                       "plain_kernel_ms_by_pattern": {p: round(v, 1) for p, v in sorted(plain_ms.items())},
                        "speedup_vs_plain": round(t_plain / t_batch, 3), "counts_equal_plain": True}}
 
 
@@ -220,8 +252,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only the counting kernels of one step (for rocprofv3 counter runs)")
     ap.add_argument("--workload", choices=["cliques", "batch", "serve"], default="cliques")
-    ap.add_argument("--batch-graphs", type=int, default=256)
-    ap.add_argument("--batch-max-n", type=int, default=65536)
+    ap.add_argument("--batch-graphs", type=int, default=2048)
+    ap.add_argument("--batch-max-n", type=int, default=262144)
     ap.add_argument("--serve-min-scale", type=int, default=10)
     ap.add_argument("--serve-max-scale", type=int, default=16)
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 2000])
