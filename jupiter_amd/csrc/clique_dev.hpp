@@ -364,12 +364,6 @@ __device__ __forceinline__ uint64_t edge_masked_sum(const RowT& R, const int16_t
   return s;
 }
 
-// 16 positions of row `row` in 32-position chunk q, the half given by sh.
-template <class RowT>
-__device__ __forceinline__ uint32_t row_slice(const RowT& R, int row, int w, int sh) {
-  return row >= 0 ? (uint32_t)((R(row, w) >> sh) & 0xffffull) : 0u;
-}
-
 // Operand row with a single 1 at position y (when y lies in the 16
 // positions of chunk q's half `half`): the A side of E_Y * R_X^T, whose entry
 // [y][x] is bit y of R[x], i.e. the edge x -> y.
@@ -416,15 +410,25 @@ __device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, i
     const int y = ay < n ? (int)J[ay] : -1;
     const int jf = uni(J[32 * Y]);
     const int q0 = max(2 * wlo, (jf + 1) >> 5);  // R[y] is empty up to the block's first member
+    const int yr = y >= 0 ? y : 0;
+    const uint32_t ym16 = y >= 0 ? 0xffffu : 0u;
     for (int X = 0; X <= Y; ++X) {
       const int bx = 32 * X + col;
       const int x = bx < n ? (int)J[bx] : -1;
+      const int xr = x >= 0 ? x : 0;
+      const uint32_t xm16 = x >= 0 ? 0xffffu : 0u;
       v16i32 acc = zero16();
+      // loads of chunk q + 1 are issued before chunk q's table reads and MFMA
+      uint64_t ri = R(i, q0 >> 1), ry = R(yr, q0 >> 1), rx = R(xr, q0 >> 1);
       for (int q = q0; q <= qe; ++q) {
-        const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
-        const uint32_t ab = y >= 0 ? (uint32_t)(((R(y, w) & R(i, w)) >> sh) & 0xffffull) : 0u;
-        const v4i32 bo = bits_to_bytes(lut, row_slice(R, x, w, sh));
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lut, ab), bo, acc, 0, 0, 0);
+        const int sh = 32 * (q & 1) + 16 * half;
+        const uint32_t ab = (uint32_t)((ry & ri) >> sh) & ym16;
+        const uint32_t bb = (uint32_t)(rx >> sh) & xm16;
+        const int wn = min(q + 1, qe) >> 1;
+        ri = R(i, wn);
+        ry = R(yr, wn);
+        rx = R(xr, wn);
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lut, ab), bits_to_bytes(lut, bb), acc, 0, 0, 0);
       }
       s += edge_masked_sum(R, J, n, Y, x, half, acc);
     }
@@ -467,14 +471,28 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, in
       const int x0 = member(X0), x1 = hx1 ? member(X1) : -1;
       v16i32 c00 = zero16(), c01 = zero16(), c10 = zero16(), c11 = zero16();
       v16i32 m00 = zero16(), m01 = zero16(), m10 = zero16(), m11 = zero16();
+      // rows absent from the tile read row 0 and are masked to zero, so every
+      // load is unconditional; chunk q + 1's row words are loaded before
+      // chunk q's table reads and MFMAs
+      const int ry0 = y0 >= 0 ? y0 : 0, ry1 = y1 >= 0 ? y1 : 0, rx0 = x0 >= 0 ? x0 : 0, rx1 = x1 >= 0 ? x1 : 0;
+      const uint32_t my0 = y0 >= 0 ? 0xffffu : 0u, my1 = y1 >= 0 ? 0xffffu : 0u;
+      const uint32_t mx0 = x0 >= 0 ? 0xffffu : 0u, mx1 = x1 >= 0 ? 0xffffu : 0u;
+      uint64_t wi = R(i, q0 >> 1), wy0 = R(ry0, q0 >> 1), wy1 = R(ry1, q0 >> 1);
+      uint64_t wx0 = R(rx0, q0 >> 1), wx1 = R(rx1, q0 >> 1);
       for (int q = q0; q <= qe; ++q) {
-        const int w = q >> 1, sh = 32 * (q & 1) + 16 * half;
-        const uint64_t ri = R(i, w);
+        const int sh = 32 * (q & 1) + 16 * half;
         const bool in0 = q >= ma0 && q <= mb0, in1 = q >= ma1 && q <= mb1;
-        const v4i32 a0 = bits_to_bytes(lut, y0 >= 0 ? (uint32_t)(((R(y0, w) & ri) >> sh) & 0xffffull) : 0u);
-        const v4i32 b0 = bits_to_bytes(lut, row_slice(R, x0, w, sh));
+        const uint32_t sa0 = (uint32_t)((wy0 & wi) >> sh) & my0, sa1 = (uint32_t)((wy1 & wi) >> sh) & my1;
+        const uint32_t sb0 = (uint32_t)(wx0 >> sh) & mx0, sb1 = (uint32_t)(wx1 >> sh) & mx1;
+        const int wn = min(q + 1, qe) >> 1;
+        wi = R(i, wn);
+        wy0 = R(ry0, wn);
+        wy1 = R(ry1, wn);
+        wx0 = R(rx0, wn);
+        wx1 = R(rx1, wn);
+        const v4i32 a0 = bits_to_bytes(lut, sa0), a1 = bits_to_bytes(lut, sa1);
+        const v4i32 b0 = bits_to_bytes(lut, sb0), b1 = bits_to_bytes(lut, sb1);
         c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-        const v4i32 b1 = bits_to_bytes(lut, row_slice(R, x1, w, sh));  // zero when the tile has no X1 block
         if (v01) c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
         if (in0) {
           const v4i32 e0 = one_hot16(y0, q, half);
@@ -482,7 +500,6 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, in
           if (v01) m01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e0, b1, m01, 0, 0, 0);
         }
         if (hy1) {
-          const v4i32 a1 = bits_to_bytes(lut, y1 >= 0 ? (uint32_t)(((R(y1, w) & ri) >> sh) & 0xffffull) : 0u);
           c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
           if (hx1) c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
           if (in1) {
