@@ -17,6 +17,9 @@
  *                                  the reference's host chunk scheduler,
  *                                  common/scheduler.cc, is never called)
  *   jgpm_serve_*                   no reference counterpart (continuous batching)
+ *   jgpm_graph_from_edges /
+ *   jgpm_graph_rmat                graph construction + orientation on the GPU
+ *                                  (common/graph.cc:8-124,210-401 do it on the host)
  *   jgpm_gen_* / jgpm_csr_*        host graph construction used by the CLI, tests
  *                                  and benchmarks; orientation follows
  *                                  Graph::orientation (common/graph.cc:210-401)
@@ -151,6 +154,24 @@ typedef struct {
 } jgpm_tuning;
 int  jgpm_get_tuning(jgpm_ctx_t ctx, jgpm_tuning* t);
 int  jgpm_set_tuning(jgpm_ctx_t ctx, const jgpm_tuning* t);
+
+/* ---- on-device ingest ---------------------------------------------------- */
+/* Undirected edge list (host arrays of m endpoint pairs, ids in [0, nv)) built
+ * into a resident graph on the GPU: self loops dropped, both directions,
+ * ascending duplicate-free lists (device radix sort), and the degree DAG
+ * (rule of common/graph.cc:229-230).  The same graph as jgpm_csr_from_edges
+ * followed by jgpm_graph_upload(JGPM_SYM).  Requires m < 2^30. */
+int  jgpm_graph_from_edges(jgpm_ctx_t ctx, int32_t nv, int64_t m, const int32_t* src,
+                           const int32_t* dst, jgpm_graph_t* out);
+/* Seeded R-MAT sampled and built on the GPU: the same graph as jgpm_gen_rmat
+ * (identical samples) uploaded with JGPM_SYM.  Requires edge_factor << scale
+ * < 2^30. */
+int  jgpm_graph_rmat(jgpm_ctx_t ctx, int scale, int edge_factor, double a, double b, double c,
+                     uint64_t seed, jgpm_graph_t* out);
+/* Copy a resident graph's symmetric (dag = 0) or DAG (dag = 1) CSR back to a
+ * host CSR handle (free with jgpm_csr_free; jgpm_csr_write saves it in the
+ * reference's file format). */
+int  jgpm_graph_download(jgpm_graph_t g, int dag, jgpm_csr_t* out);
 
 /* ---- batching: many (graph, pattern, range) queries in one launch ------ */
 typedef struct {

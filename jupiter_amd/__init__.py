@@ -104,6 +104,10 @@ def lib():
         "jgpm_set_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
         "jgpm_count_batch": ([P, ctypes.POINTER(_Query), ctypes.c_int, ctypes.POINTER(U64),
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_graph_from_edges": ([P, I32, I64, P, P, PP], ctypes.c_int),
+        "jgpm_graph_rmat": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, U64,
+                             PP], ctypes.c_int),
+        "jgpm_graph_download": ([P, ctypes.c_int, PP], ctypes.c_int),
         "jgpm_batch_create": ([P, ctypes.POINTER(_Query), ctypes.c_int, PP], ctypes.c_int),
         "jgpm_batch_run": ([P, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "jgpm_batch_free": ([P], ctypes.c_int),
@@ -299,6 +303,21 @@ class Context:
     def upload(self, csr, flags=SYM):
         return Graph(self, csr, flags)
 
+    def graph_from_edges(self, nv, src, dst):
+        """Build a resident graph on the GPU from an undirected edge list (int32 arrays)."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().jgpm_graph_from_edges(self._h, nv, len(src), _ptr(src), _ptr(dst), ctypes.byref(h)),
+               "graph_from_edges")
+        return Graph._wrap(self, h)
+
+    def rmat_graph(self, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+        """Seeded R-MAT sampled and built on the GPU (the same graph as rmat(...) uploaded)."""
+        h = ctypes.c_void_p()
+        _check(lib().jgpm_graph_rmat(self._h, scale, edge_factor, a, b, c, seed, ctypes.byref(h)), "graph_rmat")
+        return Graph._wrap(self, h)
+
     def load(self, prefix, dag_only=False):
         g = Graph.__new__(Graph)
         g.ctx = self
@@ -374,6 +393,19 @@ class Graph:
         self._h = ctypes.c_void_p()
         _check(lib().jgpm_graph_upload(ctx._h, _ptr(csr.rowptr), _ptr(csr.col), csr.nv, csr.nnz, flags,
                                        ctypes.byref(self._h)), "graph_upload")
+
+    @classmethod
+    def _wrap(cls, ctx, handle):
+        g = cls.__new__(cls)
+        g.ctx = ctx
+        g._h = handle
+        return g
+
+    def download(self, dag=False):
+        """The resident symmetric (or DAG) CSR copied back to the host as a CSR."""
+        h = ctypes.c_void_p()
+        _check(lib().jgpm_graph_download(self._h, int(dag), ctypes.byref(h)), "graph_download")
+        return CSR._from_handle(h)
 
     def free(self):
         if self._h:

@@ -206,16 +206,18 @@ using namespace jgpm;
 // Probabilities become 32-bit integer thresholds, so sampling is integer-only
 // and bit-reproducible across machines.
 
+RmatCuts jgpm::rmat_cuts(double a, double b, double c) {
+  const double T = 4294967296.0;
+  return RmatCuts{(uint64_t)std::floor(a * T), (uint64_t)std::floor((a + b) * T), (uint64_t)std::floor((a + b + c) * T)};
+}
+
 extern "C" int jgpm_gen_rmat(int scale, int edge_factor, double a, double b, double c,
                              uint64_t seed, jgpm_csr_t* out) {
   if (!out || scale < 1 || scale > 30 || edge_factor < 1) return JGPM_E_INVALID;
   if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return JGPM_E_INVALID;
   const int32_t nv = (int32_t)(1u << scale);
   const int64_t m = (int64_t)edge_factor << scale;
-  const uint64_t T = 4294967296ull;
-  const uint64_t ta = (uint64_t)std::floor(a * (double)T);
-  const uint64_t tab = (uint64_t)std::floor((a + b) * (double)T);
-  const uint64_t tabc = (uint64_t)std::floor((a + b + c) * (double)T);
+  const RmatCuts t = rmat_cuts(a, b, c);
   std::vector<int32_t> src, dst;
   try {
     src.resize((size_t)m);
@@ -225,20 +227,8 @@ extern "C" int jgpm_gen_rmat(int scale, int edge_factor, double a, double b, dou
   }
   parallel_for(m, 1 << 15, [&](int64_t bgn, int64_t end) {
     for (int64_t i = bgn; i < end; ++i) {
-      uint64_t st = mix_seed(seed, (uint64_t)i);
-      uint32_t u = 0, v = 0;
-      uint64_t bits = 0;
-      for (int l = 0; l < scale; ++l) {
-        if ((l & 1) == 0) bits = splitmix64(st);
-        uint64_t r = (l & 1) ? (bits >> 32) : (bits & 0xffffffffull);
-        uint32_t bu, bv;
-        if (r < ta) { bu = 0; bv = 0; }
-        else if (r < tab) { bu = 0; bv = 1; }
-        else if (r < tabc) { bu = 1; bv = 0; }
-        else { bu = 1; bv = 1; }
-        u = (u << 1) | bu;
-        v = (v << 1) | bv;
-      }
+      uint32_t u, v;
+      rmat_sample(scale, t, seed, (uint64_t)i, u, v);
       src[i] = (int32_t)u;
       dst[i] = (int32_t)v;
     }

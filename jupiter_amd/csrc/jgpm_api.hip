@@ -111,7 +111,7 @@ static void free_csr(DevCSR& d) {
 
 // Root tier lists of the DAG: vertex ids (ascending) whose out-degree falls
 // in each block tier; the wave tier scans the root range directly.
-static int build_tiers(jgpm_graph_s* g, const int64_t* rp, int32_t nv) {
+int build_tiers(jgpm_graph_s* g, const int64_t* rp, int32_t nv) {
   for (int t = 0; t < jk::NUM_TIERS; ++t) g->tier_h[t].clear();
   for (int32_t v = 0; v < nv; ++v) {
     int64_t d = rp[v + 1] - rp[v];
@@ -131,6 +131,21 @@ static int build_tiers(jgpm_graph_s* g, const int64_t* rp, int32_t nv) {
     HIP_TRY(hipMemcpy(g->tier_d[t], g->tier_h[t].data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
     g->bytes += (int64_t)(n * sizeof(int32_t));
   }
+  return JGPM_OK;
+}
+
+// Topological order of the degree DAG used by the clique engine's ranking:
+// position of every vertex in (symmetric degree, id) order, by counting sort.
+int set_degree_order(jgpm_graph_s* g, const std::vector<int64_t>& sym_rp, int32_t max_deg) {
+  const int32_t nv = (int32_t)sym_rp.size() - 1;
+  std::vector<int32_t> ord((size_t)std::max(nv, 1));
+  std::vector<int64_t> pos((size_t)max_deg + 2, 0);
+  for (int32_t v = 0; v < nv; ++v) pos[sym_rp[v + 1] - sym_rp[v] + 1]++;
+  for (int32_t k = 0; k <= max_deg; ++k) pos[k + 1] += pos[k];
+  for (int32_t v = 0; v < nv; ++v) ord[v] = (int32_t)pos[sym_rp[v + 1] - sym_rp[v]]++;
+  if (hipMalloc(&g->d_ord, sizeof(int32_t) * std::max(nv, 1)) != hipSuccess) return JGPM_E_NOMEM;
+  if (hipMemcpy(g->d_ord, ord.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice) != hipSuccess) return JGPM_E_HIP;
+  g->bytes += 4 * (int64_t)nv;
   return JGPM_OK;
 }
 
@@ -173,14 +188,14 @@ static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_
   {
     // a topological order of the DAG: (degree, id) when the symmetric degrees
     // are known (the orientation rule), else Kahn's algorithm on the DAG
-    std::vector<int32_t> ord((size_t)dag->nv);
     if (sym) {
-      int32_t md = sym->max_deg;
-      std::vector<int64_t> pos((size_t)md + 2, 0);
-      for (int32_t v = 0; v < sym->nv; ++v) pos[sym->rowptr[v + 1] - sym->rowptr[v] + 1]++;
-      for (int32_t k = 0; k <= md; ++k) pos[k + 1] += pos[k];
-      for (int32_t v = 0; v < sym->nv; ++v) ord[v] = (int32_t)pos[sym->rowptr[v + 1] - sym->rowptr[v]]++;
-    } else {
+      st = set_degree_order(g, sym->rowptr, sym->max_deg);
+      if (st) { jgpm_graph_free(g); return st; }
+      *out = g;
+      return JGPM_OK;
+    }
+    std::vector<int32_t> ord((size_t)dag->nv);
+    {
       std::vector<int32_t> indeg((size_t)dag->nv, 0), q;
       q.reserve((size_t)dag->nv);
       for (int64_t k = 0; k < dag->nnz; ++k) indeg[dag->col[k]]++;

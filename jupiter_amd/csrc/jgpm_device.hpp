@@ -81,6 +81,9 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
                    unsigned* qctr_base = nullptr, uint64_t* gscr = nullptr, size_t gscr_words = 0);
 // Blocks of the global-row tier one launch may use, and their row scratch.
 int glb_tier_grid(const jgpm_ctx_s* c);
+// Root tier lists of the DAG (host row pointers) and the degree order.
+int build_tiers(jgpm_graph_s* g, const int64_t* rp, int32_t nv);
+int set_degree_order(jgpm_graph_s* g, const std::vector<int64_t>& sym_rp, int32_t max_deg);
 int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st);
 // Batched persistent launch over queries the kernel supports (TC, P1..P16);
 // counts land in c->d_bcnt[0..n).  hq / cs are host staging that must stay

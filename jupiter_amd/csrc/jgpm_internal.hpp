@@ -26,16 +26,43 @@ int host_threads();
 void parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& f);
 
 // splitmix64: the integer PRNG behind every generator (portable, seedable,
-// counter-based so results do not depend on the thread count).
-static inline uint64_t splitmix64(uint64_t& x) {
+// counter-based so results do not depend on the thread count).  The same
+// functions run in the device generators (ingest.hip), so a graph generated
+// on the GPU is bit-identical to the host one.
+#ifdef __HIP__
+#define JGPM_HD __host__ __device__
+#else
+#define JGPM_HD
+#endif
+JGPM_HD static inline uint64_t splitmix64(uint64_t& x) {
   uint64_t z = (x += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
-static inline uint64_t mix_seed(uint64_t seed, uint64_t stream) {
+JGPM_HD static inline uint64_t mix_seed(uint64_t seed, uint64_t stream) {
   uint64_t s = seed ^ (stream * 0xD1B54A32D192ED03ull);
   return splitmix64(s);
+}
+// R-MAT quadrant thresholds: probabilities as 32-bit integer cut points.
+struct RmatCuts {
+  uint64_t ta, tab, tabc;
+};
+RmatCuts rmat_cuts(double a, double b, double c);
+// One R-MAT sample (vertex bits drawn from sample i's own stream).
+JGPM_HD static inline void rmat_sample(int scale, const RmatCuts& t, uint64_t seed, uint64_t i, uint32_t& u,
+                                       uint32_t& v) {
+  uint64_t st = mix_seed(seed, i);
+  u = 0;
+  v = 0;
+  uint64_t bits = 0;
+  for (int l = 0; l < scale; ++l) {
+    if ((l & 1) == 0) bits = splitmix64(st);
+    const uint64_t r = (l & 1) ? (bits >> 32) : (bits & 0xffffffffull);
+    const uint32_t bu = r >= t.tab, bv = (r >= t.ta && r < t.tab) || r >= t.tabc;
+    u = (u << 1) | bu;
+    v = (v << 1) | bv;
+  }
 }
 
 int build_sym_csr(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst, jgpm_csr_s& out);
