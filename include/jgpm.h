@@ -17,6 +17,9 @@
  *                                  the reference's host chunk scheduler,
  *                                  common/scheduler.cc, is never called)
  *   jgpm_serve_*                   no reference counterpart (continuous batching)
+ *   jgpm_plan_*, jgpm_count_plan   the general pattern generator the reference leaves as
+ *                                  a TODO (src/gpu_kernels_codegen.cuh:3-4); levels as
+ *                                  in PatternConfig (src/config.h:39-58)
  *   jgpm_graph_from_edges /
  *   jgpm_graph_rmat                graph construction + orientation on the GPU
  *                                  (common/graph.cc:8-124,210-401 do it on the host)
@@ -172,6 +175,27 @@ int  jgpm_graph_rmat(jgpm_ctx_t ctx, int scale, int edge_factor, double a, doubl
  * host CSR handle (free with jgpm_csr_free; jgpm_csr_write saves it in the
  * reference's file format). */
 int  jgpm_graph_download(jgpm_graph_t g, int dag, jgpm_csr_t* out);
+
+/* ---- general patterns: pattern graph -> compiled plan -> count ---------- */
+/* Compile a connected pattern with k (2..7) vertices and the given edges into
+ * a matching plan: matching order, adjacency (and, with induced != 0,
+ * non-adjacency) constraints per level, and symmetry-breaking restrictions
+ * from the pattern's automorphism group, so that every occurrence is matched
+ * once.  induced = 0 counts subgraphs (the P-id semantics), 1 counts
+ * vertex-induced subgraphs.  JGPM_E_PATTERN if the pattern is not connected. */
+typedef struct jgpm_plan_s* jgpm_plan_t;
+int  jgpm_plan_compile(int32_t k, int32_t nedges, const int32_t* eu, const int32_t* ev, int induced,
+                       jgpm_plan_t* out);
+/* order[l] (k entries): pattern vertex matched at level l; aut_size = |Aut|;
+ * restrictions (ra[i], rb[i]) mean v_ra < v_rb over levels.  *nrestrict is
+ * the capacity on entry and the count on return.  Any pointer may be NULL. */
+int  jgpm_plan_describe(jgpm_plan_t p, int32_t* order, int64_t* aut_size, int32_t* ra, int32_t* rb,
+                        int32_t* nrestrict);
+/* Count occurrences on the symmetric CSR; a root range selects the vertices
+ * matched at level 0 (so disjoint ranges add up to the total). */
+int  jgpm_count_plan(jgpm_graph_t g, jgpm_plan_t p, int64_t root_begin, int64_t root_end,
+                     uint64_t* count, double* kernel_ms);
+int  jgpm_plan_free(jgpm_plan_t p);
 
 /* ---- batching: many (graph, pattern, range) queries in one launch ------ */
 typedef struct {

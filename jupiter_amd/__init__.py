@@ -104,6 +104,11 @@ def lib():
         "jgpm_set_tuning": ([P, ctypes.POINTER(_Tuning)], ctypes.c_int),
         "jgpm_count_batch": ([P, ctypes.POINTER(_Query), ctypes.c_int, ctypes.POINTER(U64),
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_plan_compile": ([I32, I32, P, P, ctypes.c_int, PP], ctypes.c_int),
+        "jgpm_plan_describe": ([P, ctypes.POINTER(I32), ctypes.POINTER(I64), ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                ctypes.POINTER(I32)], ctypes.c_int),
+        "jgpm_count_plan": ([P, P, I64, I64, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "jgpm_plan_free": ([P], ctypes.c_int),
         "jgpm_graph_from_edges": ([P, I32, I64, P, P, PP], ctypes.c_int),
         "jgpm_graph_rmat": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, U64,
                              PP], ctypes.c_int),
@@ -357,6 +362,39 @@ class Context:
         return {f: getattr(t, f) for f, _ in _Tuning._fields_}, ms.value
 
 
+class Plan:
+    """A compiled general pattern (jgpm_plan_*): k vertices, edges [(a, b), ...]."""
+
+    def __init__(self, k, edges, induced=False):
+        eu = np.ascontiguousarray([a for a, _ in edges], dtype=np.int32)
+        ev = np.ascontiguousarray([b for _, b in edges], dtype=np.int32)
+        self.k = k
+        self._h = ctypes.c_void_p()
+        _check(lib().jgpm_plan_compile(k, len(edges), _ptr(eu), _ptr(ev), int(induced), ctypes.byref(self._h)),
+               "plan_compile")
+
+    def describe(self):
+        """(matching order, |Aut|, restrictions [(a, b)] meaning v_a < v_b over levels)."""
+        order = (ctypes.c_int32 * self.k)()
+        aut = ctypes.c_int64()
+        ra = (ctypes.c_int32 * 64)()
+        rb = (ctypes.c_int32 * 64)()
+        n = ctypes.c_int32(64)
+        _check(lib().jgpm_plan_describe(self._h, order, ctypes.byref(aut), ra, rb, ctypes.byref(n)), "plan_describe")
+        return list(order), aut.value, [(ra[i], rb[i]) for i in range(n.value)]
+
+    def free(self):
+        if self._h:
+            lib().jgpm_plan_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Batch:
     """Prepared batched count (jgpm_batch_*): run() replays the captured launches."""
 
@@ -400,6 +438,14 @@ class Graph:
         g.ctx = ctx
         g._h = handle
         return g
+
+    def count_plan(self, plan, roots=None):
+        """Count a compiled general pattern; roots select the level-0 vertices. Returns (count, kernel_ms)."""
+        rb, re = (0, -1) if roots is None else roots
+        out = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        _check(lib().jgpm_count_plan(self._h, plan._h, rb, re, ctypes.byref(out), ctypes.byref(ms)), "count_plan")
+        return int(out.value), ms.value
 
     def download(self, dag=False):
         """The resident symmetric (or DAG) CSR copied back to the host as a CSR."""

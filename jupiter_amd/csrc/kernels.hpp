@@ -1,5 +1,6 @@
 // Device kernel entry points of libjgpm (host-callable launchers).
 #pragma once
+#include "plan.hpp"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -62,6 +63,20 @@ struct SymArgs {
   uint64_t* out;
 };
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
+
+// One wave per root vertex v_0 of [rb, re) on the symmetric CSR.
+struct PlanArgs {
+  const int64_t* rp;
+  const int32_t* ci;
+  int64_t rb, re;
+  unsigned long long* qctr;
+  int32_t* scr;                  // per wave (k + 1) lists of `stride` ints
+  int64_t stride;                // >= maximum degree
+  uint64_t* out;
+  Plan plan;
+};
+constexpr int PLAN_WPB = 4;
+hipError_t launch_plan(const PlanArgs& a, int grid, hipStream_t st);
 int sym_waves_per_block();
 int sym_lists_per_wave();
 hipError_t launch_aux_src(const int64_t* rp, int32_t nv, int32_t* src, int grid, hipStream_t st);
