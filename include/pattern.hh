@@ -27,6 +27,8 @@ class Pattern {
   int get_max_degree() const { return max_degree; }
   const std::vector<vidType>& get_neighbours(vidType v) const { return adj_list.at(v); }
   const std::vector<vidType>& N(vidType v) const { return adj_list.at(v); }
+  // vertex -> neighbour list of the pattern (for plan compilation)
+  const std::map<vidType, std::vector<vidType>>& adjacency() const { return adj_list; }
   bool is_connected(vidType u, vidType v) const;
   Pattern& add_edge(vidType u, vidType v);
   Pattern& remove_edge(vidType u, vidType v);

@@ -1,7 +1,9 @@
 // jupiter_gpm — drop-in command line of the reference (src/main.cc:7-48):
 //   jupiter_gpm <graph_prefix> <pattern> [total_round=1]
 // <pattern> is P1..P16 (or TC, STAR3, TAILTRI, I_C4, I_DIAMOND, I_TAILTRI,
-// I_STAR3, I_K4, or a pattern edge-list file).  <graph_prefix> names the
+// I_STAR3, I_K4, or a pattern edge-list file: one isomorphic to a catalogue
+// pattern runs its kernel, any other connected pattern of up to 7 vertices
+// runs as a compiled plan).  <graph_prefix> names the
 // .meta.txt/.vertex.bin/.edge.bin files; "rmat:<scale>[:<seed>]" generates a
 // seeded R-MAT graph (edge factor 16, a/b/c = 0.57/0.19/0.19) instead.
 // P13..P16 load the degree DAG, as in the reference.  The final lines match
@@ -11,8 +13,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "graph.h"
 #include "jgpm.h"
@@ -36,18 +40,39 @@ static bool dag_pattern(int p) { return p == JGPM_TC || (p >= JGPM_P13 && p <= J
 int SubgraphSolver(Graph& g, uint64_t& total, int argc, char* argv[]) {
   std::string pname(argv[2]);
   int pid = pattern_from_name(pname);
+  jgpm_plan_t plan = nullptr;  // a pattern file outside the catalogue runs as a compiled plan
   if (pid < 0) {
     Pattern pat(pname, false);
     pid = pat.pattern_id();
     if (pid < 0) {
-      std::cerr << "pattern " << pname << " is not in the catalogue\n";
-      return JGPM_E_PATTERN;
+      std::map<vidType, int32_t> id;
+      for (const auto& kv : pat.adjacency()) id.emplace(kv.first, (int32_t)id.size());
+      std::vector<int32_t> eu, ev;
+      for (const auto& kv : pat.adjacency())
+        for (vidType u : kv.second)
+          if (kv.first < u) {
+            eu.push_back(id.at(kv.first));
+            ev.push_back(id.at(u));
+          }
+      const int st = jgpm_plan_compile((int32_t)id.size(), (int32_t)eu.size(), eu.data(), ev.data(), 0, &plan);
+      if (st) {
+        std::cerr << "pattern " << pname << ": " << jgpm_strerror(st) << "\n";
+        return st;
+      }
+      if (g.is_dag()) {
+        jgpm_plan_free(plan);
+        std::cerr << "pattern " << pname << " needs the symmetric graph\n";
+        return JGPM_E_GRAPHKIND;
+      }
     }
   }
   int total_round = argc >= 4 ? std::max(1, atoi(argv[3])) : 1;
   jgpm_ctx_t ctx = nullptr;
   int st = jgpm_init(0, &ctx);
-  if (st) return st;
+  if (st) {
+    jgpm_plan_free(plan);
+    return st;
+  }
   jgpm_graph_t dg = nullptr;
   st = jgpm_graph_upload(ctx, g.out_rowptr(), g.out_colidx(), g.num_vertices(), g.num_edges(),
                          g.is_dag() ? JGPM_DAG : JGPM_SYM, &dg);
@@ -64,7 +89,7 @@ int SubgraphSolver(Graph& g, uint64_t& total, int argc, char* argv[]) {
     if (end < begin) end = begin;
     uint64_t c = 0;
     double ms = 0;
-    st = jgpm_count(dg, pid, begin, end, &c, &ms);
+    st = plan ? jgpm_count_plan(dg, plan, begin, end, &c, &ms) : jgpm_count(dg, pid, begin, end, &c, &ms);
     total += c;
     seconds += ms * 1e-3;
     begin = end;
@@ -77,6 +102,7 @@ int SubgraphSolver(Graph& g, uint64_t& total, int argc, char* argv[]) {
   }
   jgpm_graph_free(dg);
   jgpm_shutdown(ctx);
+  if (plan) jgpm_plan_free(plan);
   return st;
 }
 
