@@ -319,7 +319,10 @@ def main():
         # dominant kernel and its work-model bytes
         (dp, di, dname), dms = max(kern.items(), key=lambda kv: sum(kv[1]))
         split = G.work_bytes_split(dp, (0, q))
-        dbytes = split[di]
+        # the work model runs the fused per-tier kernels: match the dominant
+        # counting launch to its tier by name
+        by_name = dict(zip([n for n, _ in ctx.last_profile()], split))
+        dbytes = by_name[dname]
         avg_ms = float(np.mean(dms))
         achieved = dbytes / (avg_ms * 1e-3) / 1e9
         total_bytes = {p: G.work_bytes(p, (0, q)) for p in QUERIES}

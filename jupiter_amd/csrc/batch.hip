@@ -495,13 +495,16 @@ extern "C" int jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* q, int n, jgp
   // memsets happen inside launch_batch; profiling events stay out)
   const bool prof = ctx->prof_on;
   ctx->prof_on = false;
+  ctx->capturing = true;
   if (hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     ctx->prof_on = prof;
+    ctx->capturing = false;
     return fail(JGPM_E_HIP);
   }
   st = launch_batch(ctx, B->q.data(), n, B->hq, B->b, B->grid, ctx->st);
   const hipError_t ce = hipStreamEndCapture(ctx->st, &B->graph);
   ctx->prof_on = prof;
+  ctx->capturing = false;
   if (st) return fail(st);
   if (ce != hipSuccess || !B->graph) return fail(JGPM_E_HIP);
   if (hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) return fail(JGPM_E_HIP);

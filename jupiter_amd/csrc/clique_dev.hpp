@@ -536,7 +536,7 @@ __device__ __forceinline__ int hfind16(const int16_t* h16, const int32_t* L, int
 // LDS layout of one root slot.  The build-phase arrays (segment offsets,
 // list bases, long-list degrees) and the count-phase arrays (compacted row
 // lists, deferred rows, the compressed bitmap) share one region.
-template <int G, int WMAX, int K, bool BYTES, bool LROWS>
+template <int G, int WMAX, int K, bool BYTES, bool LROWS, bool CNT = true>
 struct Lay {
   static constexpr int DM = 64 * WMAX;
   static constexpr bool ROWS = K >= 4;
@@ -571,7 +571,7 @@ struct Lay {
   static constexpr size_t C_SCW = al(C_SC + (MULTI ? (size_t)8 * G * 64 : 0));  // u32 [G][64]
   static constexpr size_t C_WCB = C_SCW;
   static constexpr size_t C_END = al(C_WCB + ((MULTI && BYTES) ? (size_t)4 * NC : 0));
-  static constexpr size_t U_SIZE = B_END > C_END ? B_END : C_END;
+  static constexpr size_t U_SIZE = (!CNT || B_END > C_END) ? B_END : C_END;  // CNT: the count view is used
   static constexpr size_t SLOT = al(U_OFF + U_SIZE);
 };
 
@@ -580,11 +580,14 @@ struct Lay {
 // with BYTES, its work-model bytes) to `acc`.  `ar` is the group's LDS
 // arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes; `lut` is the block's
 // fill_bit_lut table (read by the 5-clique matrix-core count only).
-template <int G, int WMAX, int K, bool BYTES, bool LROWS>
+template <int G, int WMAX, int K, bool BYTES, bool LROWS, int PHASE = 0>
 __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                              const int32_t* __restrict__ ord, int64_t b0, int d, int ratio, unsigned char* ar,
                              uint64_t* rowp, int32_t* s_wtot, uint64_t& acc, const uint64_t* lut) {
-  using LY = Lay<G, WMAX, K, BYTES, LROWS>;
+  // PHASE 0: build and count; 1: build only (rows left in rowp); 2: count
+  // only (rowp already holds the rows of this root)
+  static_assert(PHASE == 0 || (!BYTES && K >= 4), "split phases are for counting with rows");
+  using LY = Lay<G, WMAX, K, BYTES, LROWS, PHASE != 1>;
   constexpr int DM = LY::DM;
   constexpr bool NEED_ROWS = LY::ROWS;
   const int r = Grp<G>::rank();
@@ -604,6 +607,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   constexpr bool RANK = LY::MULTI;
   const int W = (d + 63) >> 6;
   const RowLayout lay(W);
+  if constexpr (PHASE != 2) {
   if constexpr (BYTES) {
     if (r == 0) acc += 16 + 4 * (uint64_t)d;
   }
@@ -745,7 +749,8 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   Grp<G>::sync();
   if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  if constexpr (K == 3) return;
+  }  // build
+  if constexpr (K == 3 || PHASE == 1) return;
   // ---- count (the build-phase region U is now free)
   if constexpr (!LY::MULTI) {
     // one word per row, lane i owns row i

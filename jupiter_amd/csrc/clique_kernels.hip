@@ -85,6 +85,95 @@ clique_kernel(CliqueArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Split form of the 1024-wide tier: the build runs with 16 waves per root
+// (twice the waves that the count's registers and per-wave lists allow) and
+// leaves each root's rows in a global slot; the count kernel reloads them.
+// Task t (of a.ids) uses slot t of a.gscr, ROWW words each.
+constexpr int SPLIT_WMAX = 16;
+constexpr size_t ROWW = (size_t)64 * SPLIT_WMAX * SPLIT_WMAX;
+
+template <int K>
+__global__ void __launch_bounds__(64 * 16) clique_build_kernel(CliqueArgs a) {
+  constexpr int G = 16;
+  using LY = Lay<G, SPLIT_WMAX, K, false, true, false>;
+  __shared__ __attribute__((aligned(16))) unsigned char s_arena[LY::SLOT];
+  __shared__ int32_t s_wtot[G + 1];
+  __shared__ int64_t s_task;
+  uint64_t* rowp = reinterpret_cast<uint64_t*>(s_arena + LY::R_OFF);
+  uint64_t acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = (int64_t)atomicAdd(a.qctr, 1u);
+    __syncthreads();
+    const int64_t task = s_task;
+    __syncthreads();
+    if (task >= a.ntasks) break;
+    const int32_t v0 = a.ids[task];
+    const int64_t b0 = a.rp[v0];
+    const int d = (int)(a.rp[v0 + 1] - b0);
+    if (d < a.dmin || d > a.dmax || d > LY::DM) continue;
+    process_root<G, SPLIT_WMAX, K, false, true, 1>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc,
+                                                   nullptr);
+    const int n = d * RowLayout((d + 63) >> 6).S;
+    uint64_t* dst = a.gscr + (size_t)task * ROWW;
+    for (int t = threadIdx.x; t < n; t += 64 * G) dst[t] = rowp[t];
+    __syncthreads();
+  }
+}
+
+template <int K>
+__global__ void __launch_bounds__(64 * 8) clique_count_kernel(CliqueArgs a) {
+  constexpr int G = 8;
+  using LY = Lay<G, SPLIT_WMAX, K, false, true, true>;
+  __shared__ __attribute__((aligned(16))) unsigned char s_arena[LY::SLOT];
+  __shared__ int32_t s_wtot[G + 1];
+  __shared__ int64_t s_task;
+  __shared__ uint64_t s_lut[K == 5 ? 256 : 1];
+  if constexpr (K == 5) {
+    fill_bit_lut(s_lut, (int)threadIdx.x, 64 * G);
+    __syncthreads();
+  }
+  uint64_t* rowp = reinterpret_cast<uint64_t*>(s_arena + LY::R_OFF);
+  uint64_t acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = (int64_t)atomicAdd(a.qctr, 1u);
+    __syncthreads();
+    const int64_t task = s_task;
+    __syncthreads();
+    if (task >= a.ntasks) break;
+    const int32_t v0 = a.ids[task];
+    const int64_t b0 = a.rp[v0];
+    const int d = (int)(a.rp[v0 + 1] - b0);
+    if (d < a.dmin || d > a.dmax || d > LY::DM) continue;
+    const int n = d * RowLayout((d + 63) >> 6).S;
+    const uint64_t* src = a.gscr + (size_t)task * ROWW;
+    for (int t = threadIdx.x; t < n; t += 64 * G) rowp[t] = __builtin_nontemporal_load(src + t);
+    __syncthreads();
+    process_root<G, SPLIT_WMAX, K, false, true, 2>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc,
+                                                   s_lut);
+  }
+  block_add<64 * G>(acc, a.out);
+}
+
+template <int K>
+static hipError_t launch_split_k(bool build, const CliqueArgs& a, int grid, hipStream_t st) {
+  if (build) hipLaunchKernelGGL(clique_build_kernel<K>, dim3(grid), dim3(64 * 16), 0, st, a);
+  else hipLaunchKernelGGL(clique_count_kernel<K>, dim3(grid), dim3(64 * 8), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_clique_split(int k, bool build, const CliqueArgs& a, int grid, hipStream_t st) {
+  switch (k) {
+    case 4: return launch_split_k<4>(build, a, grid, st);
+    case 5: return launch_split_k<5>(build, a, grid, st);
+    case 6: return launch_split_k<6>(build, a, grid, st);
+    case 7: return launch_split_k<7>(build, a, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+size_t split_slot_words() { return ROWW; }
+
+// ---------------------------------------------------------------------------
 // Host-side launch table.
 template <int G, int WPB, int WMAX, int K, bool BYTES, bool LROWS>
 static hipError_t launch_one(const CliqueArgs& a, int grid, hipStream_t st) {

@@ -476,6 +476,30 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
       a.ntasks = n;
       a.chunk = 1;
       grid = (int)std::min<int64_t>(maxgrid, n);
+      if (tier == jk::TIER_LDS16 && !bytes && k >= 4 && !gscr && !c->capturing) {
+        // split form: build with 16 waves per root into global row slots,
+        // then count; chunks of roots bound the slots to 1 GiB
+        const size_t slot = jk::split_slot_words();
+        const int64_t per = std::max<int64_t>(1, ((int64_t)1 << 27) / (int64_t)slot);
+        const int64_t cap = std::min<int64_t>(n, per);
+        int s2 = ensure_scratch(c, slot * (size_t)cap);
+        if (s2) return s2;
+        for (int64_t b = 0; b < n; b += cap) {
+          jk::CliqueArgs s = a;
+          s.ids = a.ids + b;
+          s.ntasks = std::min<int64_t>(cap, n - b);
+          s.gscr = c->d_scr;
+          const int sg = (int)std::min<int64_t>(c->num_cu, s.ntasks);
+          for (int part = 0; part < 2; ++part) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, a.qctr, 1, 0u);
+            prof_begin(c, st, part == 0 ? "clique_lds1024_build" : kTierName[tier]);
+            hipError_t e = jk::launch_clique_split(k, part == 0, s, sg, st);
+            prof_end(c, st);
+            if (e != hipSuccess) return JGPM_E_HIP;
+          }
+        }
+        continue;
+      }
       size_t sw = jk::tier_scratch_words(tier);
       if (sw) {
         grid = (int)std::min<int64_t>(grid, glb_tier_grid(c));

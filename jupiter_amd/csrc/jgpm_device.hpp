@@ -51,6 +51,7 @@ struct jgpm_ctx_s {
   const char* pname[MAX_PROF] = {nullptr};
   int nprof = 0;
   bool prof_on = true;
+  bool capturing = false;  // launches are being captured into a HIP graph: nothing may allocate
   std::mutex mu;
 };
 

@@ -47,6 +47,10 @@ struct BQ {
 };
 
 hipError_t launch_clique(int k, bool bytes, int tier, const CliqueArgs& a, int grid, hipStream_t st);
+// Split form of TIER_LDS16 for counting (k >= 4): build kernel (rows to
+// a.gscr slots of split_slot_words() words per task), then count kernel.
+hipError_t launch_clique_split(int k, bool build, const CliqueArgs& a, int grid, hipStream_t st);
+size_t split_slot_words();
 
 // Symmetric-graph patterns P1..P12: one wave per directed edge of [e0, e0+ne).
 struct SymArgs {
