@@ -40,16 +40,15 @@ constexpr int BWPB = 4;  // independent waves per block
 
 template <int K>
 __device__ __forceinline__ void batch_root(const int64_t* rp, const int32_t* ci, const int32_t* ord, int64_t b0,
-                                           int d, int ratio, unsigned char* ar, int32_t* wtot, uint64_t& acc,
-                                           const uint64_t* lut) {
+                                           int d, int ratio, unsigned char* ar, int32_t* wtot, uint64_t& acc) {
   if (d <= 64) {
     using LY = Lay<1, 1, K, false, true>;
     process_root<1, 1, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
-                                       wtot, acc, lut);
+                                       wtot, acc);
   } else {
     using LY = Lay<1, 2, K, false, true>;
     process_root<1, 2, K, false, true>(rp, ci, ord, b0, d, ratio, ar, reinterpret_cast<uint64_t*>(ar + LY::R_OFF),
-                                       wtot, acc, lut);
+                                       wtot, acc);
   }
 }
 
@@ -78,9 +77,6 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
   constexpr int ARENA = ((CL > SY ? CL : SY) + 15) & ~15;
   __shared__ __attribute__((aligned(16))) unsigned char s_arena[BWPB][ARENA];
   __shared__ int32_t s_wtot[BWPB][2];
-  __shared__ uint64_t s_lut[256];
-  fill_bit_lut(s_lut, (int)threadIdx.x, 64 * BWPB);
-  __syncthreads();
   const int wv = threadIdx.x >> 6;
   const int ln = lane_id();
   unsigned char* ar = s_arena[wv];
@@ -110,11 +106,11 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
         const int d = (int)(Q.rp[v0 + 1] - b0);
         if (d < Q.kind - 1 || d > a.hub) continue;
         switch (Q.kind) {
-          case 3: batch_root<3>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
-          case 4: batch_root<4>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
-          case 5: batch_root<5>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
-          case 6: batch_root<6>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
-          case 7: batch_root<7>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc, s_lut); break;
+          case 3: batch_root<3>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 4: batch_root<4>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 5: batch_root<5>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 6: batch_root<6>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
+          case 7: batch_root<7>(Q.rp, Q.ci, Q.ord, b0, d, a.ratio, ar, s_wtot[wv], acc); break;
         }
       } else {
         const int64_t e = Q.base + u;

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -200,10 +201,22 @@ struct RowLayout {
   }
   __device__ int operator()(int i, int w) const { return i * S + (w ^ ((i >> sh) & X)); }
 };
+// Row accessors: word w of row i, and its 32-bit half h (bits 32h .. 32h + 31).
+struct RowsLds {
+  uint64_t* p;
+  RowLayout lay;
+  __device__ uint64_t operator()(int i, int w) const { return p[lay(i, w)]; }
+  __device__ uint32_t half(int i, int w, int h) const {
+    return reinterpret_cast<const uint32_t*>(p)[2 * lay(i, w) + h];
+  }
+};
 struct RowsGlobal {
   uint64_t* p;
   RowLayout lay;
   __device__ uint64_t operator()(int i, int w) const { return __builtin_nontemporal_load(p + lay(i, w)); }
+  __device__ uint32_t half(int i, int w, int h) const {
+    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p) + 2 * lay(i, w) + h);
+  }
 };
 
 // Block reduction of a per-thread uint64 and one atomicAdd per block.
@@ -307,29 +320,44 @@ __device__ uint64_t recR(const uint64_t (&c)[WM], int wlo, int nw, const RowT& R
   }
 }
 
-typedef int v4i32 __attribute__((ext_vector_type(4)));
-typedef int v16i32 __attribute__((ext_vector_type(16)));
+typedef int v8i32 __attribute__((ext_vector_type(8)));
+typedef float v16f32 __attribute__((ext_vector_type(16)));
 
-// Byte -> 8 bytes of 0/1 (bit e -> byte e), the table behind bits_to_bytes:
-// each nibble is spread by one 24-bit multiply (n * 0x204081 puts bit k at
-// 8k, no carries) and masked.  Threads t, t + nt, ... of the block fill it.
-__device__ __forceinline__ void fill_bit_lut(uint64_t* lut, int t, int nt) {
-  for (int c = t; c < 256; c += nt) {
-    const uint64_t lo = ((uint32_t)(c & 15) * 0x00204081u) & 0x01010101u;
-    const uint64_t hi = ((uint32_t)(c >> 4) * 0x00204081u) & 0x01010101u;
-    lut[c] = lo | (hi << 32);
-  }
+// The 5-clique count runs on the fp4 (e2m1) matrix cores: 0/1 operands are
+// exact in e2m1 and the f32 accumulators hold the counts (at most 2^11 here)
+// exactly.  v_mfma_scale_f32_32x32x64_f8f6f4 with fp4 operands covers 64
+// positions per instruction from 16 operand bytes per lane: twice the
+// positions per cycle of the int8 instructions and half their operand bytes.
+//
+// Operand encoding, in registers: bit e = 4n + d of a lane's 32 positions is
+// element e of the operand (dword d, nibble n), so a set bit only has to be
+// masked, or moved inside its nibble.  The A side keeps bits 0..2 of each
+// nibble where they are (e2m1 0.5, 1.0, 2.0) and moves bit 3 to bit 0 (0.5);
+// the B side holds the reciprocal at every element (2.0, 1.0, 0.5, 2.0), so
+// each product of two set elements is exactly 1 and a product counts common
+// positions.
+__device__ __forceinline__ v8i32 enc_a(uint32_t x) {
+  v8i32 v;
+  v[0] = (int)(x & 0x11111111u);
+  v[1] = (int)(x & 0x22222222u);
+  v[2] = (int)(x & 0x44444444u);
+  v[3] = (int)((x >> 3) & 0x11111111u);
+  v[4] = v[5] = v[6] = v[7] = 0;  // the fp4 form reads the first four dwords
+  return v;
+}
+__device__ __forceinline__ v8i32 enc_b(uint32_t x) {
+  v8i32 v;
+  v[0] = (int)((x << 2) & 0x44444444u);
+  v[1] = (int)(x & 0x22222222u);
+  v[2] = (int)((x >> 2) & 0x11111111u);
+  v[3] = (int)((x >> 1) & 0x44444444u);
+  v[4] = v[5] = v[6] = v[7] = 0;
+  return v;
 }
 
-// 16 bits -> 16 bytes of 0/1 (bit e -> byte e): two lookups in the LDS table.
-__device__ __forceinline__ v4i32 bits_to_bytes(const uint64_t* lut, uint32_t b) {
-  const uint64_t lo = lut[b & 255u], hi = lut[(b >> 8) & 255u];
-  v4i32 v;
-  v[0] = (int)(uint32_t)lo;
-  v[1] = (int)(uint32_t)(lo >> 32);
-  v[2] = (int)(uint32_t)hi;
-  v[3] = (int)(uint32_t)(hi >> 32);
-  return v;
+// D += A B^T on 32 x 32 x 64 fp4 operands with unit scales (E8M0 127 = 2^0).
+__device__ __forceinline__ v16f32 mma_fp4(const v8i32& a, const v8i32& b, const v16f32& c) {
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
 }
 
 // Matrix-core instructions run on the whole wave whatever EXEC holds, so
@@ -338,183 +366,168 @@ __device__ __forceinline__ v4i32 bits_to_bytes(const uint64_t* lut, uint32_t b) 
 // passed through readfirstlane.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ v16i32 zero16() {
-  v16i32 z;
+__device__ __forceinline__ v16f32 zero16() {
+  v16f32 z;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) z[k] = 0;
+  for (int k = 0; k < 16; ++k) z[k] = 0.f;
   return z;
 }
 
 // Sum of the accumulator entries [y][x] of block Y against column member x
-// whose pair is a DAG edge x -> y (bit y of R[x]), read from the rows.
+// whose pair is a DAG edge x -> y (bit y of R[x]).  Lane (col, half) holds
+// the entries of members ym = 32Y + 8g + 4half + e (g, e < 4): four runs of
+// four consecutive J entries, read as 8-byte words.  Row x is read at the
+// block's first two words; members further on (rare) read their own word.
+// Entries are counts below 2^11, so the f32 sum of 16 is exact.
 template <class RowT>
-__device__ __forceinline__ uint64_t edge_masked_sum(const RowT& R, const int16_t* J, int n, int Y, int x, int half,
-                                                    const v16i32& acc) {
-  uint64_t s = 0;
+__device__ __forceinline__ uint32_t edge_masked_sum(const RowT& R, const int16_t* J, int n, int whi, int Y, int x,
+                                                    int half, const v16f32& acc) {
+  float s = 0.f;
   if (x >= 0) {
+    const int w0 = uni(J[32 * Y]) >> 6;
+    const uint64_t r0 = R(x, w0), r1 = R(x, min(w0 + 1, whi));
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int ym = 32 * Y + 8 * (a >> 2) + 4 * half + (a & 3);
-      if (ym < n) {
-        const int yy = J[ym];
-        if ((R(x, yy >> 6) >> (yy & 63)) & 1ull) s += (uint64_t)(uint32_t)acc[a];
+    for (int g = 0; g < 4; ++g) {
+      const int base = 32 * Y + 8 * g + 4 * half;
+      const uint64_t four = *reinterpret_cast<const uint64_t*>(J + base);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int yy = (int)(int16_t)(uint16_t)(four >> (16 * e));
+        if (base + e < n) {
+          const int w = yy >> 6;
+          const uint64_t word = w == w0 ? r0 : (w == w0 + 1 ? r1 : R(x, w));
+          if ((word >> (yy & 63)) & 1ull) s += acc[4 * g + e];
+        }
       }
     }
   }
-  return s;
+  return (uint32_t)s;
 }
 
-// Operand row with a single 1 at position y (when y lies in the 16
-// positions of chunk q's half `half`): the A side of E_Y * R_X^T, whose entry
+// A-side operand with a single set element at position y (when y lies in
+// the 32 positions of word q's half `half`): E_Y in E_Y * R_X^T, whose entry
 // [y][x] is bit y of R[x], i.e. the edge x -> y.
-__device__ __forceinline__ v4i32 one_hot16(int y, int q, int half) {
-  v4i32 v;
-  const bool in = y >= 0 && (y >> 4) == 2 * q + half;
-  const int e = y & 15;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = (in && (e >> 2) == k) ? (1 << (8 * (e & 3))) : 0;
-  return v;
+__device__ __forceinline__ v8i32 one_hot_fp4(int y, int q, int half) {
+  const bool in = y >= 0 && (y >> 5) == 2 * q + half;
+  return enc_a(in ? 1u << (y & 31) : 0u);
 }
 
-__device__ __forceinline__ uint64_t dot16(const v16i32& c, const v16i32& m) {
-  uint32_t s = 0;
+// Entries are counts below 2^11 and 0/1 masks: the f32 sum of 16 products
+// is exact.
+__device__ __forceinline__ uint32_t dot16(const v16f32& c, const v16f32& m) {
+  float s = 0.f;
 #pragma unroll
-  for (int a = 0; a < 16; ++a) s += (uint32_t)c[a] * (uint32_t)m[a];
-  return s;
+  for (int a = 0; a < 16; ++a) s += c[a] * m[a];
+  return (uint32_t)s;
 }
 
 // Triangles of the DAG induced on S = R[i] (rows ranked in DAG order, so
-// every row only has bits above its own position), by int8 matrix cores:
+// every row only has bits above its own position), on the matrix cores:
 //   sum over member pairs x -> y of |R[y] ∩ R[i] ∩ R[x]|
 // = sum over 32x32 blocks (X <= Y) of  mask(x -> y) . (A_Y B_X^T)[y][x],
 // where A_Y holds the rows R[y] ∩ R[i] of the Y block's members and B_X the
-// rows R[x] of the X block's members, both over the row's word window in
-// chunks of 32 positions (one v_mfma_i32_32x32x32_i8 per chunk).  Lane l
-// supplies member 32B + l%32 and the 16 positions 16 * (l / 32) .. +15 of
-// the chunk; accumulator element a of lane l is entry
-// [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  J lists S's positions in
+// rows R[x] of the X block's members, over the row's words from the first
+// that can hold a common successor to whi, one fp4 MFMA per 64-position word.
+// Lane l supplies member 32B + l%32 and the 32 positions 32 * (l / 32) .. +31
+// of the word; accumulator element a of lane l is entry
+// [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  Absent members (past n) read row
+// zr, which is empty (the last row of the DAG order), so every load and
+// product is unconditional; the first word's product starts from a zero C
+// operand, so the accumulators need no clearing.  J lists S's positions in
 // ascending order (n > 0 entries); every lane of the wave must call this.
-// This form keeps one accumulator per block pair (narrow roots, where the
-// registers of the tiled form below would cost occupancy).
+// This form keeps one accumulator per block pair and reads the edge mask from
+// the rows (narrow roots, where registers decide occupancy).
 template <class RowT>
-__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_,
-                          const uint64_t* lut) {
-  const int n = uni(n_), i = uni(i_), wlo = uni(wlo_), whi = uni(whi_);
+__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int whi_, int zr_) {
+  const int n = uni(n_), i = uni(i_), whi = uni(whi_), zr = uni(zr_);
   const int ln = lane_id();
   const int col = ln & 31, half = ln >> 5;
   const int T = (n + 31) >> 5;
-  const int qe = 2 * whi + 1;
   uint64_t s = 0;
   for (int Y = 0; Y < T; ++Y) {
     const int ay = 32 * Y + col;
     const int y = ay < n ? (int)J[ay] : -1;
-    const int jf = uni(J[32 * Y]);
-    const int q0 = max(2 * wlo, (jf + 1) >> 5);  // R[y] is empty up to the block's first member
-    const int yr = y >= 0 ? y : 0;
-    const uint32_t ym16 = y >= 0 ? 0xffffu : 0u;
+    const int q0 = (uni(J[32 * Y]) + 1) >> 6;  // R[y] is empty up to the block's first member
+    if (q0 > whi) continue;
+    const int ry = y >= 0 ? y : zr;
     for (int X = 0; X <= Y; ++X) {
       const int bx = 32 * X + col;
       const int x = bx < n ? (int)J[bx] : -1;
-      const int xr = x >= 0 ? x : 0;
-      const uint32_t xm16 = x >= 0 ? 0xffffu : 0u;
-      v16i32 acc = zero16();
-      // loads of chunk q + 1 are issued before chunk q's table reads and MFMA
-      uint64_t ri = R(i, q0 >> 1), ry = R(yr, q0 >> 1), rx = R(xr, q0 >> 1);
-      for (int q = q0; q <= qe; ++q) {
-        const int sh = 32 * (q & 1) + 16 * half;
-        const uint32_t ab = (uint32_t)((ry & ri) >> sh) & ym16;
-        const uint32_t bb = (uint32_t)(rx >> sh) & xm16;
-        const int wn = min(q + 1, qe) >> 1;
-        ri = R(i, wn);
-        ry = R(yr, wn);
-        rx = R(xr, wn);
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lut, ab), bits_to_bytes(lut, bb), acc, 0, 0, 0);
-      }
-      s += edge_masked_sum(R, J, n, Y, x, half, acc);
+      const int rx = x >= 0 ? x : zr;
+      // word q + 1's row halves are loaded before word q's table reads
+      uint32_t wi = R.half(i, q0, half), wy = R.half(ry, q0, half), wx = R.half(rx, q0, half);
+      auto step = [&](int q, const v16f32& c) -> v16f32 {
+        const uint32_t sa = wy & wi, sb = wx;
+        const int wn = min(q + 1, whi);
+        wi = R.half(i, wn, half);
+        wy = R.half(ry, wn, half);
+        wx = R.half(rx, wn, half);
+        return mma_fp4(enc_a(sa), enc_b(sb), c);
+      };
+      v16f32 acc = step(q0, zero16());
+      for (int q = q0 + 1; q <= whi; ++q) acc = step(q, acc);
+      s += edge_masked_sum(R, J, n, whi, Y, x, half, acc);
     }
   }
   return s;
 }
 
-// The same count with 2 x 2 tiles of block pairs (widest roots); the edge
-// mask is the product E_Y R_X^T of one-hot rows, accumulated on the chunks
-// that hold the Y block's members.
+// The same count with 2 x 2 tiles of block pairs (widest roots): each word's
+// four operand slices feed four independent count accumulators.  The edge
+// mask is the product E_Y R_X^T of one-hot rows at the Y members' own
+// positions, accumulated on the words that hold them (q0 .. qm).  Pairs of
+// the tile outside the triangle X <= Y, or with absent blocks, have an empty
+// mask or empty operands, so they add zero.
 template <class RowT>
-__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, int wlo_, int whi_,
-                         const uint64_t* lut) {
-  const int n = uni(n_), i = uni(i_), wlo = uni(wlo_), whi = uni(whi_);
+__device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, int whi_, int zr_) {
+  const int n = uni(n_), i = uni(i_), whi = uni(whi_), zr = uni(zr_);
   const int ln = lane_id();
   const int col = ln & 31, half = ln >> 5;
   const int T = (n + 31) >> 5;
-  const int qe = 2 * whi + 1;
   auto member = [&](int B) -> int {
     const int a = 32 * B + col;
-    return (B < T && a < n) ? (int)J[a] : -1;
+    return a < n ? (int)J[a] : -1;
   };
   uint64_t s = 0;
-  // 2 x 2 tiles of block pairs: each chunk's four operand slices feed four
-  // independent count accumulators and, on the chunks holding the Y blocks'
-  // own members, four edge-mask accumulators
   for (int Y0 = 0; Y0 < T; Y0 += 2) {
-    const int Y1 = Y0 + 1;
-    const bool hy1 = Y1 < T;
-    const int y0 = member(Y0), y1 = member(Y1);
-    const int jf0 = uni(J[32 * Y0]);
-    const int ma0 = jf0 >> 5, mb0 = uni(J[min(n - 1, 32 * Y0 + 31)]) >> 5;
-    const int ma1 = hy1 ? uni(J[32 * Y1]) >> 5 : 1 << 30, mb1 = hy1 ? uni(J[min(n - 1, 32 * Y1 + 31)]) >> 5 : -1;
-    const int q0 = min(ma0, max(2 * wlo, (jf0 + 1) >> 5));  // rows of the tile are empty before this chunk
-    const int Xend = hy1 ? Y1 : Y0;
+    const int y0 = member(Y0), y1 = member(Y0 + 1);
+    const int ry0 = y0 >= 0 ? y0 : zr, ry1 = y1 >= 0 ? y1 : zr;
+    // words q0 .. qm hold the members of the two Y blocks
+    const int q0 = uni(J[32 * Y0]) >> 6, qm = uni(J[min(n - 1, 32 * Y0 + 63)]) >> 6;
+    const int Xend = min(Y0 + 1, T - 1);
     for (int X0 = 0; X0 <= Xend; X0 += 2) {
-      const int X1 = X0 + 1;
-      const bool hx1 = X1 <= Xend;
-      const bool v01 = hx1 && X1 <= Y0;  // block pair (Y0, X1) has edges x -> y only if X1 <= Y0
-      const int x0 = member(X0), x1 = hx1 ? member(X1) : -1;
-      v16i32 c00 = zero16(), c01 = zero16(), c10 = zero16(), c11 = zero16();
-      v16i32 m00 = zero16(), m01 = zero16(), m10 = zero16(), m11 = zero16();
-      // rows absent from the tile read row 0 and are masked to zero, so every
-      // load is unconditional; chunk q + 1's row words are loaded before
-      // chunk q's table reads and MFMAs
-      const int ry0 = y0 >= 0 ? y0 : 0, ry1 = y1 >= 0 ? y1 : 0, rx0 = x0 >= 0 ? x0 : 0, rx1 = x1 >= 0 ? x1 : 0;
-      const uint32_t my0 = y0 >= 0 ? 0xffffu : 0u, my1 = y1 >= 0 ? 0xffffu : 0u;
-      const uint32_t mx0 = x0 >= 0 ? 0xffffu : 0u, mx1 = x1 >= 0 ? 0xffffu : 0u;
-      uint64_t wi = R(i, q0 >> 1), wy0 = R(ry0, q0 >> 1), wy1 = R(ry1, q0 >> 1);
-      uint64_t wx0 = R(rx0, q0 >> 1), wx1 = R(rx1, q0 >> 1);
-      for (int q = q0; q <= qe; ++q) {
-        const int sh = 32 * (q & 1) + 16 * half;
-        const bool in0 = q >= ma0 && q <= mb0, in1 = q >= ma1 && q <= mb1;
-        const uint32_t sa0 = (uint32_t)((wy0 & wi) >> sh) & my0, sa1 = (uint32_t)((wy1 & wi) >> sh) & my1;
-        const uint32_t sb0 = (uint32_t)(wx0 >> sh) & mx0, sb1 = (uint32_t)(wx1 >> sh) & mx1;
-        const int wn = min(q + 1, qe) >> 1;
-        wi = R(i, wn);
-        wy0 = R(ry0, wn);
-        wy1 = R(ry1, wn);
-        wx0 = R(rx0, wn);
-        wx1 = R(rx1, wn);
-        const v4i32 a0 = bits_to_bytes(lut, sa0), a1 = bits_to_bytes(lut, sa1);
-        const v4i32 b0 = bits_to_bytes(lut, sb0), b1 = bits_to_bytes(lut, sb1);
-        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-        if (v01) c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
-        if (in0) {
-          const v4i32 e0 = one_hot16(y0, q, half);
-          m00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e0, b0, m00, 0, 0, 0);
-          if (v01) m01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e0, b1, m01, 0, 0, 0);
+      const int x0 = member(X0), x1 = X0 + 1 <= Xend ? member(X0 + 1) : -1;
+      const int rx0 = x0 >= 0 ? x0 : zr, rx1 = x1 >= 0 ? x1 : zr;
+      v16f32 c00, c01, c10, c11, m00, m01, m10, m11;
+      // word q + 1's row halves are loaded before word q's table reads
+      uint32_t wi = R.half(i, q0, half), wy0 = R.half(ry0, q0, half), wy1 = R.half(ry1, q0, half);
+      uint32_t wx0 = R.half(rx0, q0, half), wx1 = R.half(rx1, q0, half);
+      auto step = [&](int q, bool first, bool with_mask) {
+        const uint32_t sa0 = wy0 & wi, sa1 = wy1 & wi, sb0 = wx0, sb1 = wx1;
+        const int wn = min(q + 1, whi);
+        wi = R.half(i, wn, half);
+        wy0 = R.half(ry0, wn, half);
+        wy1 = R.half(ry1, wn, half);
+        wx0 = R.half(rx0, wn, half);
+        wx1 = R.half(rx1, wn, half);
+        const v8i32 a0 = enc_a(sa0), a1 = enc_a(sa1);
+        const v8i32 b0 = enc_b(sb0), b1 = enc_b(sb1);
+        const v16f32 z = zero16();
+        c00 = mma_fp4(a0, b0, first ? z : c00);
+        c01 = mma_fp4(a0, b1, first ? z : c01);
+        c10 = mma_fp4(a1, b0, first ? z : c10);
+        c11 = mma_fp4(a1, b1, first ? z : c11);
+        if (with_mask) {
+          const v8i32 e0 = one_hot_fp4(y0, q, half), e1 = one_hot_fp4(y1, q, half);
+          m00 = mma_fp4(e0, b0, first ? z : m00);
+          m01 = mma_fp4(e0, b1, first ? z : m01);
+          m10 = mma_fp4(e1, b0, first ? z : m10);
+          m11 = mma_fp4(e1, b1, first ? z : m11);
         }
-        if (hy1) {
-          c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
-          if (hx1) c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
-          if (in1) {
-            const v4i32 e1 = one_hot16(y1, q, half);
-            m10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e1, b0, m10, 0, 0, 0);
-            if (hx1) m11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(e1, b1, m11, 0, 0, 0);
-          }
-        }
-      }
-      s += dot16(c00, m00);
-      if (v01) s += dot16(c01, m01);
-      if (hy1) {
-        s += dot16(c10, m10);
-        if (hx1) s += dot16(c11, m11);
-      }
+      };
+      step(q0, true, true);  // word q0 holds the first Y member
+      for (int q = q0 + 1; q <= whi; ++q) step(q, false, q <= qm);
+      s += (uint64_t)dot16(c00, m00) + dot16(c01, m01) + dot16(c10, m10) + dot16(c11, m11);
     }
   }
   return s;
@@ -578,12 +591,11 @@ struct Lay {
 // Process one root v0 (list N+(v0) = ci[b0 .. b0 + d)) with a group of G
 // waves: build the bitmap rows in `rowp` and add the root's k-cliques (or,
 // with BYTES, its work-model bytes) to `acc`.  `ar` is the group's LDS
-// arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes; `lut` is the block's
-// fill_bit_lut table (read by the 5-clique matrix-core count only).
+// arena of Lay<G, WMAX, K, BYTES, LROWS>::SLOT bytes.
 template <int G, int WMAX, int K, bool BYTES, bool LROWS, int PHASE = 0>
 __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                              const int32_t* __restrict__ ord, int64_t b0, int d, int ratio, unsigned char* ar,
-                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc, const uint64_t* lut) {
+                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc) {
   // PHASE 0: build and count; 1: build only (rows left in rowp); 2: count
   // only (rowp already holds the rows of this root)
   static_assert(PHASE == 0 || (!BYTES && K >= 4), "split phases are for counting with rows");
@@ -760,10 +772,8 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       else acc += rec1<K - 2, false>(Ri, rowp, wL);
     }
   } else {
-    auto R = [&](int i, int w) -> uint64_t {
-      if constexpr (LROWS) return rowp[lay(i, w)];
-      else return RowsGlobal{rowp, lay}(i, w);
-    };
+    using RowT = typename std::conditional<LROWS, RowsLds, RowsGlobal>::type;
+    const RowT R{rowp, lay};
     int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;
     uint64_t* Rc = reinterpret_cast<uint64_t*>(U + LY::C_SC) + (size_t)wv * 64;
     for (int i = wv; i < d; i += G) {
@@ -819,10 +829,10 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
           else acc += rec1<K - 3, false>(myrow, Rc, wL);
         }
       } else {
-        if constexpr (K == 5 && !BYTES) {
+        if constexpr (K == 5 && !BYTES && WMAX >= 4) {
           // tiles of block pairs pay for their registers only on the widest roots
-          if constexpr (WMAX >= 16) acc += tri_mfma(R, J, n, i, wlo, whi, lut);
-          else acc += tri_mfma1(R, J, n, i, wlo, whi, lut);
+          if constexpr (WMAX >= 16) acc += tri_mfma(R, J, n, i, whi, d - 1);
+          else acc += tri_mfma1(R, J, n, i, whi, d - 1);
         } else if constexpr (K == 5 && WMAX <= 16) {
           // 5-cliques: triangles inside S.  One flat loop per lane over a
           // work list shared through an LDS counter: a step either takes the

@@ -37,8 +37,23 @@ namespace jk {
 // Kernel.  G waves per root; WPB roots (groups) per block when G == 1.
 // WMAX words per row => roots with d <= 64 * WMAX.  LROWS: rows in LDS,
 // otherwise in a per-block slice of global scratch.
+// Waves per SIMD the kernel's registers are held to.  These kernels are
+// bound by latency rather than by any one unit, so on the tiers below the
+// registers are capped at the occupancy their LDS allows (measured faster on
+// the 5-clique tiers, also where the 4-word tier spills 20 bytes a lane);
+// elsewhere the compiler's own choice stands (already at the LDS or hardware
+// limit, or the cap would spill).
+template <int WMAX, int K, bool BYTES>
+constexpr int min_waves() {
+  if (BYTES) return 1;
+  if (K == 5 && WMAX == 4) return 8;
+  if (K == 5 && WMAX == 8) return 6;
+  if (K == 6 && WMAX == 4) return 7;
+  if (K == 7 && WMAX == 4) return 5;
+  return 1;
+}
 template <int G, int WPB, int WMAX, int K, bool BYTES, bool LROWS>
-__global__ void __launch_bounds__(64 * G * WPB)
+__global__ void __launch_bounds__(64 * G * WPB) __attribute__((amdgpu_waves_per_eu(min_waves<WMAX, K, BYTES>())))
 clique_kernel(CliqueArgs a) {
   using LY = Lay<G, WMAX, K, BYTES, LROWS>;
   constexpr int DM = LY::DM;
@@ -46,12 +61,6 @@ clique_kernel(CliqueArgs a) {
   __shared__ __attribute__((aligned(16))) unsigned char s_arena[WPB][LY::SLOT];
   __shared__ int32_t s_wtot[G + 1];
   __shared__ int64_t s_task;
-  constexpr bool LUT = K == 5 && !BYTES && WMAX > 1;  // the matrix-core count expands bits through a table
-  __shared__ uint64_t s_lut[LUT ? 256 : 1];
-  if constexpr (LUT) {
-    fill_bit_lut(s_lut, (int)threadIdx.x, NT);
-    __syncthreads();
-  }
   const int slot = (G == 1) ? (int)(threadIdx.x >> 6) : 0;
   const int ln = lane_id();
   unsigned char* ar = s_arena[slot];
@@ -78,7 +87,7 @@ clique_kernel(CliqueArgs a) {
       const int64_t b0 = rp[v0];
       const int d = (int)(rp[v0 + 1] - b0);
       if (d < a.dmin || d > a.dmax || d > DM) continue;
-      process_root<G, WMAX, K, BYTES, LROWS>(rp, a.ci, a.ord, b0, d, a.ratio, ar, rowp, s_wtot, acc, s_lut);
+      process_root<G, WMAX, K, BYTES, LROWS>(rp, a.ci, a.ord, b0, d, a.ratio, ar, rowp, s_wtot, acc);
     }
   }
   block_add<NT>(acc, a.out);
@@ -111,8 +120,7 @@ __global__ void __launch_bounds__(64 * 16) clique_build_kernel(CliqueArgs a) {
     const int64_t b0 = a.rp[v0];
     const int d = (int)(a.rp[v0 + 1] - b0);
     if (d < a.dmin || d > a.dmax || d > LY::DM) continue;
-    process_root<G, SPLIT_WMAX, K, false, true, 1>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc,
-                                                   nullptr);
+    process_root<G, SPLIT_WMAX, K, false, true, 1>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc);
     const int n = d * RowLayout((d + 63) >> 6).S;
     uint64_t* dst = a.gscr + (size_t)task * ROWW;
     for (int t = threadIdx.x; t < n; t += 64 * G) dst[t] = rowp[t];
@@ -127,11 +135,6 @@ __global__ void __launch_bounds__(64 * 8) clique_count_kernel(CliqueArgs a) {
   __shared__ __attribute__((aligned(16))) unsigned char s_arena[LY::SLOT];
   __shared__ int32_t s_wtot[G + 1];
   __shared__ int64_t s_task;
-  __shared__ uint64_t s_lut[K == 5 ? 256 : 1];
-  if constexpr (K == 5) {
-    fill_bit_lut(s_lut, (int)threadIdx.x, 64 * G);
-    __syncthreads();
-  }
   uint64_t* rowp = reinterpret_cast<uint64_t*>(s_arena + LY::R_OFF);
   uint64_t acc = 0;
   for (;;) {
@@ -148,8 +151,7 @@ __global__ void __launch_bounds__(64 * 8) clique_count_kernel(CliqueArgs a) {
     const uint64_t* src = a.gscr + (size_t)task * ROWW;
     for (int t = threadIdx.x; t < n; t += 64 * G) rowp[t] = __builtin_nontemporal_load(src + t);
     __syncthreads();
-    process_root<G, SPLIT_WMAX, K, false, true, 2>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc,
-                                                   s_lut);
+    process_root<G, SPLIT_WMAX, K, false, true, 2>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc);
   }
   block_add<64 * G>(acc, a.out);
 }
