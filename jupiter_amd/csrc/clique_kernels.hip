@@ -10,22 +10,24 @@
 //
 // Design (one root vertex v0 per task, L = N+(v0) sorted, d = |L|):
 //  1. Build.  L is staged in LDS.  The out-lists N+(L[i]) of all i are
-//     streamed as one flattened, coalesced sequence (a lane's element knows its
-//     list by a binary search over the LDS segment offsets); each element is
-//     looked up in L by binary search in LDS and, if present at position p,
-//     bit p of row i is set (LDS / L2 atomic OR).  Row i is then the bitmap
-//     of L ∩ N+(L[i]).  Lists much longer than d (ratio switch) are not
-//     streamed: every element of L is binary-searched in them instead
+//     streamed as one flattened, coalesced sequence (a thread's element knows
+//     its list by a forward cursor over the LDS segment offsets); each element
+//     is looked up in L (binary search for one-word roots, a 16-bit position
+//     hash otherwise) and, if present at position p, bit p of row i is set
+//     (LDS / L2 atomic OR).  Row i is then the bitmap of L ∩ N+(L[i]), in
+//     DAG order of the members.  Lists much longer than d (ratio switch) are
+//     not streamed: every element of L is binary-searched in them instead
 //     (galloping side of the merge/gallop choice).
 //  2. Count.  k-cliques with source v0 are v0 plus a (k-1)-clique of the
 //     DAG on L given by the rows.  TC needs no rows (it counts the matches of
 //     step 1).  With d <= 64 every row is one 64-bit word and lane i counts
 //     the cliques starting at i with AND + popcount.  With d > 64, the rows
-//     are multi-word; for each i the wave compacts S = row i, and either
-//     counts with word-wise AND/popcount (k = 4) or builds (per lane) the compressed
-//     |S| x |S| successor bitmap of the DAG inside S (k >= 5) and recurses
-//     on single words.
-// All arithmetic is integer; counts are exact uint64.
+//     are multi-word; for each i the wave compacts S = row i, and counts with
+//     word-wise AND/popcount (k = 4), the compressed |S| x |S| successor
+//     bitmap (k >= 5, |S| <= 64), fp4 matrix-core products (k = 5, wider S)
+//     or register word windows (k >= 6).
+// Counts are exact uint64 (the matrix-core products are exact integers
+// below 2^24 in f32).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
