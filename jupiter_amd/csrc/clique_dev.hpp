@@ -46,16 +46,22 @@ __device__ __forceinline__ int seg_search(const int* a, int n, int f) {
 }
 
 // Cursor form of seg_search for increasing f: a[i] <= f on entry; returns
-// the last j >= i with a[j] <= f (two steps forward, then a binary search).
+// the last j >= i with a[j] <= f: two single steps forward (most moves), then
+// a galloping search, so a jump over m segments costs O(log m) reads.
 __device__ __forceinline__ int seg_advance(const int* a, int n, int i, int f) {
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     if (a[i + 1] > f) return i;
     ++i;
   }
-  int lo = i, hi = n;
+  int lo = i, step = 1;
+  while (lo + step < n && a[lo + step] <= f) {
+    lo += step;
+    step <<= 1;
+  }
+  int hi = min(n, lo + step);  // a[hi] > f, or hi == n
   while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
+    const int mid = (lo + hi) >> 1;
     if (a[mid] <= f) lo = mid; else hi = mid;
   }
   return lo;
@@ -692,21 +698,24 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   if constexpr (K == 3 && BYTES) return;  // TC work model needs no build
   group_excl_scan<G>(off, d, s_wtot);
   const int T = off[d];
-  // ---- flattened stream of the light lists: thread r takes elements
-  // r, r + NT, ... (consecutive lanes read consecutive addresses); a thread's
-  // segment index only moves forward, and eight loads are in flight at once
+  // ---- flattened stream of the light lists: each wave takes one contiguous
+  // share of the elements and its lanes step through it 64 at a time
+  // (consecutive lanes read consecutive addresses).  A lane's segment index
+  // only moves forward, by the few lists 64 elements span, and eight loads
+  // are in flight at once.
   {
-    constexpr int NT = Grp<G>::size;
     constexpr int UF = 8;
-    int i = r < T ? seg_search(off, d, r) : 0;
-    for (int f0 = r; f0 < T; f0 += UF * NT) {
+    const int per = ((T + G - 1) / G + 63) & ~63;
+    const int fb = wv * per, fe = min(T, fb + per);
+    int i = fb + ln < fe ? seg_search(off, d, fb + ln) : 0;
+    for (int f0 = fb + ln; f0 < fe; f0 += UF * 64) {
       int32_t yv[UF];
       int iv[UF];
 #pragma unroll
       for (int u = 0; u < UF; ++u) {
-        const int f = f0 + u * NT;
+        const int f = f0 + u * 64;
         iv[u] = -1;
-        if (f < T) {
+        if (f < fe) {
           i = seg_advance(off, d, i, f);
           iv[u] = i;
           yv[u] = ci[ub[i] + (f - off[i])];
