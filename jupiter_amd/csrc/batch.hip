@@ -3,14 +3,15 @@
 // graph per process (src/main.cc, src/jupiter_gpm.cu:356-908).
 //
 // Work is cut into chunks: a chunk is a run of roots (clique patterns, DAG)
-// or of directed edges (P1..P12, symmetric CSR) of one query.  Every wave of
+// or of directed edges (P2..P12, symmetric CSR) of one query.  Every wave of
 // the persistent grid takes the next chunk from one device-wide atomic queue
 // (chunks of expensive queries first), processes it with the same device
 // routines as the single-query kernels, and adds the chunk's count to that
 // query's uint64 slot.  Clique roots with out-degree up to the hub threshold
 // (64 or 128, a tuning knob) run inside the wave (one- or two-word bitmap rows
 // in LDS); the few wider roots of a query run afterwards in the block-per-root
-// tier kernels, into the same slot.  All launches of a batch can be captured
+// tier kernels, into the same slot, and 4-cycle (P1) queries run afterwards as
+// wedge launches (one wave per root), also into their slots.  All launches of a batch can be captured
 // once in a HIP graph (prepared batches) and replayed.
 #include <hip/hip_runtime.h>
 
@@ -117,7 +118,6 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
         const int32_t v0 = Q.src[e], v1 = Q.ci[e];
         int64_t v = 0;
         switch (Q.kind - 100) {
-          case 1: v = batch_edge<1>(Q, w, v0, v1, lds_lists, glist, a.scr_stride); break;
           case 2: v = batch_edge<2>(Q, w, v0, v1, lds_lists, glist, a.scr_stride); break;
           case 3: v = batch_edge<3>(Q, w, v0, v1, lds_lists, glist, a.scr_stride); break;
           case 4: v = batch_edge<4>(Q, w, v0, v1, lds_lists, glist, a.scr_stride); break;
@@ -208,6 +208,15 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
       b.base = rb;
       b.ntasks = re - rb;
       b.chunk = rchunk;
+    } else if (kind == 101) {
+      // 4-cycles run as wedge launches after the persistent kernel (no chunks here)
+      int rc = ensure_p1_counters(c, (int32_t)g->sym.nv, st);
+      if (rc) return rc;
+      b.rp = g->sym.rp;
+      b.ci = g->sym.ci;
+      b.base = rb;
+      b.ntasks = 0;
+      b.chunk = 1;
     } else {
       const int p = kind - 100;
       int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
@@ -253,6 +262,13 @@ static int launch_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, const std::ve
   if (hipGetLastError() != hipSuccess) return JGPM_E_HIP;
   const int first_tier = hub <= 64 ? jk::TIER_LDS2 : jk::TIER_LDS4;
   for (int i = 0; i < n; ++i) {
+    if (hq[i].kind == 101) {  // 4-cycles: wedge launch into the query's slot
+      jgpm_graph_s* g = q[i].graph;
+      const int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->sym.nv : q[i].root_end;
+      int rc = launch_p1(g, rb, re, b.bcnt + i, b.bq32 + (size_t)jk::NUM_TIERS * i, st);
+      if (rc) return rc;
+      continue;
+    }
     if (hq[i].kind >= 100) continue;
     jgpm_graph_s* g = q[i].graph;
     int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;

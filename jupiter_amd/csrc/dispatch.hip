@@ -43,9 +43,61 @@ int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t s
 
 bool pattern_uses_tri(int p) { return p == 3 || p == 4 || p == 7 || p == 10 || p == 11; }
 
+// Waves of the 4-cycle wedge kernel on a graph of nv vertices: one dense
+// counter array of nv ints each, within 2 GiB, at most 32 waves per CU.
+static int64_t p1_waves(jgpm_ctx_s* c, int32_t nv) {
+  const int64_t budget = (int64_t)1 << 29;  // ints
+  const int64_t cap = (int64_t)c->num_cu * 32;
+  return std::max<int64_t>(jk::P1_WPB, std::min<int64_t>(cap, budget / std::max<int32_t>(nv, 1)));
+}
+
+int ensure_p1_counters(jgpm_ctx_s* c, int32_t nv, hipStream_t st) {
+  const size_t need = (size_t)p1_waves(c, nv) * (size_t)std::max<int32_t>(nv, 1);
+  if (c->p1cnt_ints >= need) return JGPM_OK;
+  if (c->capturing) return JGPM_E_STATE;
+  if (c->d_p1cnt) {
+    if (hipStreamSynchronize(st) != hipSuccess) return JGPM_E_HIP;
+    (void)hipFree(c->d_p1cnt);
+    c->d_p1cnt = nullptr;
+    c->p1cnt_ints = 0;
+  }
+  if (hipMalloc(&c->d_p1cnt, need * sizeof(int32_t)) != hipSuccess) return JGPM_E_NOMEM;
+  // zero once: every launch leaves the counters it touched at zero again
+  if (hipMemsetAsync(c->d_p1cnt, 0, need * sizeof(int32_t), st) != hipSuccess) return JGPM_E_HIP;
+  c->p1cnt_ints = need;
+  return JGPM_OK;
+}
+
+int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, hipStream_t st) {
+  jgpm_ctx_s* c = g->ctx;
+  const int32_t nv = (int32_t)g->sym.nv;
+  if (re <= rb) return JGPM_OK;
+  const int64_t waves = std::min<int64_t>(p1_waves(c, nv), (int64_t)(c->p1cnt_ints / std::max<int32_t>(nv, 1)));
+  const int grid = (int)std::min<int64_t>(waves / jk::P1_WPB, (re - rb + jk::P1_WPB - 1) / jk::P1_WPB);
+  if (grid < 1) return JGPM_E_STATE;  // counters not allocated for this graph
+  jk::P1Args a{};
+  a.rp = g->sym.rp;
+  a.ci = g->sym.ci;
+  a.rb = rb;
+  a.re = re;
+  a.nv = nv;
+  a.qctr = qctr;
+  a.cnt = c->d_p1cnt;
+  a.out = out;
+  prof_begin(c, st, "sym_p1_wedge");
+  hipError_t e = jk::launch_p1_wedge(a, grid, st);
+  prof_end(c, st);
+  return e == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+}
+
 int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStream_t st) {
   if (!g->has_sym) return JGPM_E_GRAPHKIND;
   jgpm_ctx_s* c = g->ctx;
+  if (p == 1) {
+    int rc = ensure_p1_counters(c, (int32_t)g->sym.nv, st);
+    if (rc) return rc;
+    return launch_p1(g, rb, re, c->d_out + slot, c->d_q + slot, st);
+  }
   // The triangle array is shared by later queries; its counter slot is
   // reset per query by ctx_reset_counters before it is used here.
   int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);

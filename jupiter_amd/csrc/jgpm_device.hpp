@@ -33,6 +33,8 @@ struct jgpm_ctx_s {
   unsigned long long* d_q64 = nullptr;  // CTX_SLOTS 64-bit work-queue counters
   int32_t* d_sscr = nullptr;  // list scratch of the symmetric-pattern kernels
   size_t sscr_ints = 0;
+  int32_t* d_p1cnt = nullptr;  // wedge counters of the 4-cycle kernel (all zero between launches)
+  size_t p1cnt_ints = 0;
   // batched / served launches: query table, chunk prefix, per-query counts
   void* d_bq = nullptr;
   int64_t* d_bcs = nullptr;
@@ -75,6 +77,11 @@ struct jgpm_graph_s {
 
 namespace jgpm {
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
+// 4-cycle (P1) wedge kernel: ensure_p1_counters allocates (never while
+// capturing); launch_p1 only launches, into *out, with the zeroed 32-bit
+// queue counter qctr.
+int ensure_p1_counters(jgpm_ctx_s* c, int32_t nv, hipStream_t st);
+int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, hipStream_t st);
 int ensure_scratch(jgpm_ctx_s* c, size_t words);
 int clique_k(int pattern);
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,

@@ -68,6 +68,20 @@ struct SymArgs {
 };
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
 
+// 4-cycles (P1) by wedges: one wave per root of [rb, re), counters in a
+// per-wave dense array of nv ints (cnt + wave * nv), zero before and after.
+struct P1Args {
+  const int64_t* rp;
+  const int32_t* ci;
+  int64_t rb, re;
+  int32_t nv;
+  unsigned* qctr;
+  int32_t* cnt;
+  uint64_t* out;
+};
+constexpr int P1_WPB = 4;
+hipError_t launch_p1_wedge(const P1Args& a, int grid, hipStream_t st);
+
 // One wave per root vertex v_0 of [rb, re) on the symmetric CSR.
 struct PlanArgs {
   const int64_t* rp;

@@ -146,16 +146,8 @@ __device__ int64_t edge_value(const Ctx& g, Wave& w, int32_t v0, int32_t v1, int
   int32_t* T = L[3];
   int32_t* pT = L[4];
   const int ln = lane();
-  if constexpr (P == 1) {
-    if (v1 >= v0) return 0;
-    int64_t c = 0;
-    for (int j = 0; j < d0; ++j) {
-      int32_t v2 = N0[j];
-      if (v2 >= v1) break;
-      c += w_icount(w, N1, d1, g.N(v2), g.d(v2), v0);
-    }
-    return c;
-  } else if constexpr (P == 2) {
+  static_assert(P >= 2 && P <= 12, "P1 runs as wedge counting (p1_wedge_kernel)");
+  if constexpr (P == 2) {
     if (v1 >= v0) return 0;
     return C2(w_icount(w, N0, d0, N1, d1, INT32_MAX));
   } else if constexpr (P == 3) {

@@ -28,7 +28,8 @@
 //       (s - 1 - [v3 in S])(t(v0,v3) - 1 - [v3 in S]) - (|S ∩ N(v3)| - 1)
 //   P11 sum_{S'} (a b c - q (a + b + c) + 2 q), a = s-1, b = t(v0,v2)-1, c = t(v1,v2)-1   (v0 < v1)
 //   P12 (s - 3) sum_S C(q(v2), 2)                                 (v0 < v1)
-// P1 keeps the reference nest (v2 in N(v0), v2 < v1; |N(v1) ∩ N(v2) ∩ [0, v0)|).
+// P1 runs as per-root wedge counting (p1_wedge_kernel below), which sums the
+// reference nest's per-edge values exactly.
 //
 // Set operations are wave-wide: the shorter list is the lookup side, each
 // lane binary-searches its element in the longer list (staged in LDS when it
@@ -37,6 +38,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "clique_dev.hpp"  // seg_advance, wave_sum_u64
 #include "kernels.hpp"
 #include "sym_dev.hpp"
 
@@ -82,6 +84,81 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
     for (int k = 0; k < WPB; ++k) s += s_part[k];
     if (s) atomicAdd((unsigned long long*)a.out, (unsigned long long)s);
   }
+}
+
+// ---------------------------------------------------------------------------
+// P1 (4-cycles) by wedges.  The reference's per-edge value (edges (v0, v1),
+// v1 < v0: sum over v2 in N(v0), v2 < v1, of |N(v1) ∩ N(v2) ∩ [0, v0)|;
+// src/message_passing_gpu_kernels.cuh:104-186) summed over a root's edges is
+//   sum over x < v0 of C(c_x, 2),  c_x = |{v in N(v0) : v < v0, x in N(v)}|:
+// every pair v2 < v1 of the c_x such v with the common neighbour x is one
+// term.  So a root needs one pass over its wedges (v0, v, x) with v, x < v0,
+// not an intersection per pair.  One wave per root: the first pass adds 1 to
+// c_x in the wave's dense counter array, the second takes every counter back
+// to zero with atomicExch and adds C(c, 2) for the value it took (a repeated x
+// takes 0 after its first exchange).  The wedges of 64 neighbours v at a time
+// form one flattened stream (consecutive lanes read consecutive elements).
+__global__ void __launch_bounds__(64 * P1_WPB) p1_wedge_kernel(P1Args a) {
+  __shared__ int32_t s_off[P1_WPB][65];
+  __shared__ int64_t s_lb[P1_WPB][64];
+  const int wv = threadIdx.x >> 6;
+  const int ln = lane();
+  int32_t* off = s_off[wv];
+  int64_t* lbase = s_lb[wv];
+  int32_t* cnt = a.cnt + ((int64_t)blockIdx.x * P1_WPB + wv) * a.nv;
+  uint64_t acc = 0;
+  for (;;) {
+    int64_t t = 0;
+    if (ln == 0) t = (int64_t)atomicAdd(a.qctr, 1u);
+    const int64_t r = a.rb + __shfl(t, 0);
+    if (r >= a.re) break;
+    const int32_t v0 = (int32_t)r;
+    const int64_t b0 = a.rp[v0];
+    const int m0 = lb(a.ci + b0, (int)(a.rp[v0 + 1] - b0), v0);  // neighbours below v0
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int jb = 0; jb < m0; jb += 64) {
+        const int j = jb + ln;
+        int len = 0;
+        int64_t base = 0;
+        if (j < m0) {
+          const int32_t v = a.ci[b0 + j];
+          base = a.rp[v];
+          len = lb(a.ci + base, (int)(a.rp[v + 1] - base), v0);  // x < v0
+        }
+        int incl = len;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(incl, o);
+          if (ln >= o) incl += y;
+        }
+        const int T = __shfl(incl, 63);
+        off[ln] = incl - len;
+        lbase[ln] = base - (incl - len);  // element f of v's segment is ci[lbase + f]
+        if (ln == 63) off[64] = T;
+        __syncwarp();
+        int i = 0;
+        for (int f = ln; f < T; f += 64) {
+          i = seg_advance(off, 64, i, f);
+          const int32_t x = a.ci[lbase[i] + f];
+          if (pass == 0) {
+            atomicAdd(&cnt[x], 1);
+          } else {
+            const uint64_t c = (uint32_t)atomicExch(&cnt[x], 0);
+            acc += c * (c - 1) / 2;
+          }
+        }
+        __syncwarp();
+      }
+      __threadfence();  // the increments are performed before they are taken back
+    }
+  }
+  acc = wave_sum_u64(acc);
+  if (ln == 0 && acc) atomicAdd((unsigned long long*)a.out, (unsigned long long)acc);
+}
+
+hipError_t launch_p1_wedge(const P1Args& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(p1_wedge_kernel, dim3(grid), dim3(64 * P1_WPB), 0, st, a);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -162,7 +239,6 @@ static hipError_t launch_p(const SymArgs& a, int grid, hipStream_t st) {
 
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st) {
   switch (p) {
-    case 1: return launch_p<1>(a, grid, st);
     case 2: return launch_p<2>(a, grid, st);
     case 3: return launch_p<3>(a, grid, st);
     case 4: return launch_p<4>(a, grid, st);
