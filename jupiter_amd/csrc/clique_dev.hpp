@@ -698,6 +698,11 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   if constexpr (K == 3 && BYTES) return;  // TC work model needs no build
   group_excl_scan<G>(off, d, s_wtot);
   const int T = off[d];
+  // light lists: element f of the stream sits at ci[ub[i] + f] (one LDS read
+  // per element for its address); long lists keep their own base
+  for (int t = r; t < d; t += Grp<G>::size)
+    if (!((hfl[t >> 5] >> (t & 31)) & 1u)) ub[t] -= off[t];
+  Grp<G>::sync();
   // ---- flattened stream of the light lists: each wave takes one contiguous
   // share of the elements and its lanes step through it 64 at a time
   // (consecutive lanes read consecutive addresses).  A lane's segment index
@@ -718,7 +723,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         if (f < fe) {
           i = seg_advance(off, d, i, f);
           iv[u] = i;
-          yv[u] = ci[ub[i] + (f - off[i])];
+          yv[u] = ci[ub[i] + f];
         }
       }
 #pragma unroll
