@@ -273,8 +273,8 @@ def main():
         print(json.dumps({"copy_bytes": x.numel() * 4}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the count shards by root "
-              "range; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a count shards by root range, "
+              "each GPU adding its partial count). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":

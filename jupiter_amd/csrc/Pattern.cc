@@ -88,7 +88,7 @@ std::string Pattern::to_string() const {
 }
 
 namespace {
-// Edge lists of the catalogue (vertex 0..k-1), see the P-id table in DESIGN.md.
+// Edge lists of the catalogue (vertex 0..k-1): each P-id's pattern graph H.
 struct Shape { int id; int k; std::vector<std::pair<int, int>> e; };
 std::vector<Shape> catalogue() {
   auto clique = [](int k) {

@@ -8,7 +8,7 @@ independent of the oracle's loop nests and pins them.
 import itertools
 
 # Pattern graphs H of each id (vertex labels follow the reference kernels'
-# variable names; see the pattern catalogue in DESIGN.md).
+# variable names v0..v4 of src/message_passing_gpu_kernels.cuh).
 PATTERNS = {
     0: (3, [(0, 1), (0, 2), (1, 2)]),                                   # triangle
     1: (4, [(0, 1), (1, 2), (2, 3), (3, 0)]),                           # 4-cycle
