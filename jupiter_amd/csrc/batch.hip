@@ -164,6 +164,8 @@ struct BatchBufs {
   int64_t sscr_stride = 1;
   uint64_t* gscr = nullptr;             // global-row tier scratch (nullptr: the context's)
   size_t gscr_words = 0;
+  int32_t* p1cnt = nullptr;             // zeroed wedge counters of the 4-cycle launches
+  size_t p1cnt_ints = 0;
 };
 
 __global__ void k_zero_u64(uint64_t* p, int n) {
@@ -190,11 +192,12 @@ static int batch_grid(jgpm_ctx_s* c) {
 // its chunk prefix and the list-scratch stride.  May build a graph's lazy
 // auxiliary arrays on st (kernels and allocations), so it is never captured.
 static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>& hq, std::vector<int64_t>& cs,
-                      int32_t& max_sym, hipStream_t st) {
+                      int32_t& max_sym, size_t& p1_ints, hipStream_t st) {
   const int rchunk = std::max(1, c->tune.batch_root_chunk);
   const int echunk = std::max(1, c->tune.batch_edge_chunk);
   hq.assign(n, jk::BQ{});
   max_sym = 1;
+  p1_ints = 0;
   for (int i = 0; i < n; ++i) {
     jgpm_graph_s* g = q[i].graph;
     const int kind = batch_kind(q[i].pattern);
@@ -210,8 +213,7 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
       b.chunk = rchunk;
     } else if (kind == 101) {
       // 4-cycles run as wedge launches after the persistent kernel (no chunks here)
-      int rc = ensure_p1_counters(c, (int32_t)g->sym.nv, st);
-      if (rc) return rc;
+      p1_ints = std::max(p1_ints, p1_counter_ints(c, (int32_t)g->sym.nv));
       b.rp = g->sym.rp;
       b.ci = g->sym.ci;
       b.base = rb;
@@ -265,7 +267,7 @@ static int launch_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, const std::ve
     if (hq[i].kind == 101) {  // 4-cycles: wedge launch into the query's slot
       jgpm_graph_s* g = q[i].graph;
       const int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->sym.nv : q[i].root_end;
-      int rc = launch_p1(g, rb, re, b.bcnt + i, b.bq32 + (size_t)jk::NUM_TIERS * i, st);
+      int rc = launch_p1(g, rb, re, b.bcnt + i, b.bq32 + (size_t)jk::NUM_TIERS * i, b.p1cnt, b.p1cnt_ints, st);
       if (rc) return rc;
       continue;
     }
@@ -309,8 +311,10 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
   int rc = ensure_batch_buffers(c, n);
   if (rc) return rc;
   int32_t max_sym = 1;
-  rc = plan_batch(c, q, n, hq, cs, max_sym, st);
+  size_t p1_ints = 0;
+  rc = plan_batch(c, q, n, hq, cs, max_sym, p1_ints, st);
   if (rc) return rc;
+  if (p1_ints && (rc = ensure_p1_counters(c, p1_ints, st))) return rc;
   const int grid = batch_grid(c);
   const size_t need = (size_t)grid * jk::BWPB * jk::symd::NLIST * (size_t)max_sym;
   if (c->sscr_ints < need) {
@@ -328,6 +332,8 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
   b.bq = static_cast<jk::BQ*>(c->d_bq);
   b.bcs = c->d_bcs;
   b.bcnt = c->d_bcnt;
+  b.p1cnt = c->d_p1cnt;
+  b.p1cnt_ints = c->p1cnt_ints;
   b.bq32 = c->d_bq32;
   b.qctr = c->d_q64 + (CTX_SLOTS - 2);
   b.sscr = c->d_sscr;
@@ -445,7 +451,7 @@ struct jgpm_batch_s {
 static void batch_release(jgpm_batch_s* B) {
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
-  void* bufs[] = {B->b.bq, B->b.bcs, B->b.bcnt, B->b.bq32, B->b.qctr, B->b.sscr, B->b.gscr};
+  void* bufs[] = {B->b.bq, B->b.bcs, B->b.bcnt, B->b.bq32, B->b.qctr, B->b.sscr, B->b.gscr, B->b.p1cnt};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (const jgpm_query& x : B->q) x.graph->nbatch--;
@@ -476,7 +482,8 @@ extern "C" int jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* q, int n, jgp
   };
   if (hipSetDevice(ctx->dev) != hipSuccess) return fail(JGPM_E_HIP);
   int32_t max_sym = 1;
-  st = plan_batch(ctx, B->q.data(), n, B->hq, B->cs, max_sym, ctx->st);
+  size_t p1_ints = 0;
+  st = plan_batch(ctx, B->q.data(), n, B->hq, B->cs, max_sym, p1_ints, ctx->st);
   if (st) return fail(st);
   B->grid = batch_grid(ctx);
   // global-row scratch only when a clique query has roots in that tier
@@ -496,8 +503,14 @@ extern "C" int jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* q, int n, jgp
       hipMalloc(&B->b.bcnt, sizeof(uint64_t) * n) != hipSuccess ||
       hipMalloc(&B->b.bq32, sizeof(unsigned) * jk::NUM_TIERS * n) != hipSuccess ||
       hipMalloc(&B->b.qctr, 64) != hipSuccess || hipMalloc(&B->b.sscr, sscr * sizeof(int32_t)) != hipSuccess ||
-      (glb && hipMalloc(&B->b.gscr, B->b.gscr_words * sizeof(uint64_t)) != hipSuccess))
+      (glb && hipMalloc(&B->b.gscr, B->b.gscr_words * sizeof(uint64_t)) != hipSuccess) ||
+      (p1_ints && hipMalloc(&B->b.p1cnt, p1_ints * sizeof(int32_t)) != hipSuccess))
     return fail(JGPM_E_NOMEM);
+  // the batch's own wedge counters (a later context allocation cannot move
+  // them under the captured launches): zeroed once, left zero by every replay
+  B->b.p1cnt_ints = p1_ints;
+  if (p1_ints && hipMemsetAsync(B->b.p1cnt, 0, p1_ints * sizeof(int32_t), ctx->st) != hipSuccess)
+    return fail(JGPM_E_HIP);
   if (hipMemcpyAsync(B->b.bq, B->hq.data(), sizeof(jk::BQ) * n, hipMemcpyHostToDevice, ctx->st) != hipSuccess ||
       hipMemcpyAsync(B->b.bcs, B->cs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, ctx->st) !=
           hipSuccess ||

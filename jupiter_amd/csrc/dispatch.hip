@@ -51,8 +51,9 @@ static int64_t p1_waves(jgpm_ctx_s* c, int32_t nv) {
   return std::max<int64_t>(jk::P1_WPB, std::min<int64_t>(cap, budget / std::max<int32_t>(nv, 1)));
 }
 
-int ensure_p1_counters(jgpm_ctx_s* c, int32_t nv, hipStream_t st) {
-  const size_t need = (size_t)p1_waves(c, nv) * (size_t)std::max<int32_t>(nv, 1);
+size_t p1_counter_ints(jgpm_ctx_s* c, int32_t nv) { return (size_t)p1_waves(c, nv) * (size_t)std::max<int32_t>(nv, 1); }
+
+int ensure_p1_counters(jgpm_ctx_s* c, size_t need, hipStream_t st) {
   if (c->p1cnt_ints >= need) return JGPM_OK;
   if (c->capturing) return JGPM_E_STATE;
   if (c->d_p1cnt) {
@@ -68,13 +69,14 @@ int ensure_p1_counters(jgpm_ctx_s* c, int32_t nv, hipStream_t st) {
   return JGPM_OK;
 }
 
-int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, hipStream_t st) {
+int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, int32_t* cnt, size_t cnt_ints,
+              hipStream_t st) {
   jgpm_ctx_s* c = g->ctx;
   const int32_t nv = (int32_t)g->sym.nv;
   if (re <= rb) return JGPM_OK;
-  const int64_t waves = std::min<int64_t>(p1_waves(c, nv), (int64_t)(c->p1cnt_ints / std::max<int32_t>(nv, 1)));
+  const int64_t waves = std::min<int64_t>(p1_waves(c, nv), (int64_t)(cnt_ints / std::max<int32_t>(nv, 1)));
   const int grid = (int)std::min<int64_t>(waves / jk::P1_WPB, (re - rb + jk::P1_WPB - 1) / jk::P1_WPB);
-  if (grid < 1) return JGPM_E_STATE;  // counters not allocated for this graph
+  if (!cnt || grid < 1) return JGPM_E_STATE;  // counters not sized for this graph
   jk::P1Args a{};
   a.rp = g->sym.rp;
   a.ci = g->sym.ci;
@@ -82,7 +84,7 @@ int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* 
   a.re = re;
   a.nv = nv;
   a.qctr = qctr;
-  a.cnt = c->d_p1cnt;
+  a.cnt = cnt;
   a.out = out;
   prof_begin(c, st, "sym_p1_wedge");
   hipError_t e = jk::launch_p1_wedge(a, grid, st);
@@ -94,9 +96,9 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   if (!g->has_sym) return JGPM_E_GRAPHKIND;
   jgpm_ctx_s* c = g->ctx;
   if (p == 1) {
-    int rc = ensure_p1_counters(c, (int32_t)g->sym.nv, st);
+    int rc = ensure_p1_counters(c, p1_counter_ints(c, (int32_t)g->sym.nv), st);
     if (rc) return rc;
-    return launch_p1(g, rb, re, c->d_out + slot, c->d_q + slot, st);
+    return launch_p1(g, rb, re, c->d_out + slot, c->d_q + slot, c->d_p1cnt, c->p1cnt_ints, st);
   }
   // The triangle array is shared by later queries; its counter slot is
   // reset per query by ctx_reset_counters before it is used here.

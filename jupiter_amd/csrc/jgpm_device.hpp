@@ -77,11 +77,15 @@ struct jgpm_graph_s {
 
 namespace jgpm {
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
-// 4-cycle (P1) wedge kernel: ensure_p1_counters allocates (never while
-// capturing); launch_p1 only launches, into *out, with the zeroed 32-bit
-// queue counter qctr.
-int ensure_p1_counters(jgpm_ctx_s* c, int32_t nv, hipStream_t st);
-int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, hipStream_t st);
+// 4-cycle (P1) wedge kernel.  p1_counter_ints: the zeroed counter ints a
+// graph of nv vertices needs; ensure_p1_counters grows the context's (it
+// allocates, never while capturing); launch_p1 only launches, into *out, with
+// the zeroed 32-bit queue counter qctr and counters cnt[0..cnt_ints) that are
+// all zero (and are left so).
+size_t p1_counter_ints(jgpm_ctx_s* c, int32_t nv);
+int ensure_p1_counters(jgpm_ctx_s* c, size_t need, hipStream_t st);
+int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, int32_t* cnt, size_t cnt_ints,
+              hipStream_t st);
 int ensure_scratch(jgpm_ctx_s* c, size_t words);
 int clique_k(int pattern);
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,

@@ -131,3 +131,25 @@ def test_prepared_batch_rejects(ctx):
     with pytest.raises(J.JgpmError):
         ctx.prepare_batch([])
     G.free()  # no batch holds it
+
+
+def test_prepared_batch_keeps_its_wedge_counters(ctx):
+    """4-cycle queries of a prepared batch use counters owned by the batch:
+    a later plain 4-cycle query on a larger graph (which grows the context's
+    counters) does not move them, and replays after it keep their counts."""
+    small = [ctx.upload(J.erdos_renyi(2000, 12, seed=300 + i)) for i in range(3)]
+    queries = [(G, "P1", 0, None) for G in small] + [(small[0], "P1", 100, 1500)]
+    want = _alone(queries)
+    with ctx.prepare_batch(queries) as B:
+        _say("prepared P1 batch, first replay")
+        assert B.run()[0] == want
+        big = J.erdos_renyi(200_000, 16, seed=310)
+        G = ctx.upload(big)
+        _say("plain P1 on 200k vertices")
+        assert G.count("P1")[0] == O.count(big.rowptr, big.col, J.PATTERNS["P1"])
+        for rep in range(2):
+            _say(f"replay {rep} after the context's counters grew")
+            assert B.run()[0] == want
+        G.free()
+    for G in small:
+        G.free()
