@@ -214,6 +214,10 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
     } else if (kind == 101) {
       // 4-cycles run as wedge launches after the persistent kernel (no chunks here)
       p1_ints = std::max(p1_ints, p1_counter_ints(c, (int32_t)g->sym.nv));
+      if (rb == 0 && re == g->sym.nv) {
+        int rc = ensure_rank_csr(g, st);
+        if (rc) return rc;
+      }
       b.rp = g->sym.rp;
       b.ci = g->sym.ci;
       b.base = rb;

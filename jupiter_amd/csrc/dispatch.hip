@@ -77,9 +77,13 @@ int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* 
   const int64_t waves = std::min<int64_t>(p1_waves(c, nv), (int64_t)(cnt_ints / std::max<int32_t>(nv, 1)));
   const int grid = (int)std::min<int64_t>(waves / jk::P1_WPB, (re - rb + jk::P1_WPB - 1) / jk::P1_WPB);
   if (!cnt || grid < 1) return JGPM_E_STATE;  // counters not sized for this graph
+  // a whole-graph count runs on the degree-ordered relabelling when it is
+  // built: the same identity in another vertex order gives the same total,
+  // and the hubs, ranked last, are no wedge's middle vertex
+  const bool ranked = rb == 0 && re == nv && g->rsym.ci;
   jk::P1Args a{};
-  a.rp = g->sym.rp;
-  a.ci = g->sym.ci;
+  a.rp = ranked ? g->rsym.rp : g->sym.rp;
+  a.ci = ranked ? g->rsym.ci : g->sym.ci;
   a.rb = rb;
   a.re = re;
   a.nv = nv;
@@ -98,6 +102,7 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   if (p == 1) {
     int rc = ensure_p1_counters(c, p1_counter_ints(c, (int32_t)g->sym.nv), st);
     if (rc) return rc;
+    if (rb == 0 && re == g->sym.nv && (rc = ensure_rank_csr(g, st))) return rc;
     return launch_p1(g, rb, re, c->d_out + slot, c->d_q + slot, c->d_p1cnt, c->p1cnt_ints, st);
   }
   // The triangle array is shared by later queries; its counter slot is

@@ -257,6 +257,21 @@ int build_on_device(jgpm_ctx_s* c, Scratch& S, int32_t* d_src, int32_t* d_dst, i
 
 constexpr int64_t kMaxEdges = (int64_t)1 << 30;  // 2m keys must fit the sort's int item count
 
+// ---- rank-relabelled symmetric CSR (4-cycle totals in degree order)
+__global__ void k_rank_keys(const int32_t* src, const int32_t* ci, const int32_t* ord, int64_t ne, int B,
+                            uint64_t* keys) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
+    keys[e] = ((uint64_t)(uint32_t)ord[src[e]] << B) | (uint32_t)ord[ci[e]];
+}
+__global__ void k_rank_deg(const int64_t* rp, const int32_t* ord, int32_t nv, int64_t* deg) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x)
+    deg[ord[v]] = rp[v + 1] - rp[v];
+}
+__global__ void k_low_bits(const uint64_t* keys, int64_t ne, uint64_t mask, int32_t* ci) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
+    ci[e] = (int32_t)(keys[e] & mask);
+}
+
 }  // namespace
 
 extern "C" int jgpm_graph_from_edges(jgpm_ctx_t ctx, int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
@@ -323,3 +338,60 @@ extern "C" int jgpm_graph_download(jgpm_graph_t g, int dag, jgpm_csr_t* out) {
   *out = c;
   return JGPM_OK;
 }
+
+namespace jgpm {
+
+// Relabel v -> d_ord[v]: sort the keys (ord[v], ord[w]) of every directed
+// edge, so each new list is ascending; the row of ord[v] has d(v) entries.
+int ensure_rank_csr(jgpm_graph_s* g, hipStream_t st) {
+  if (g->rsym.ci || !g->has_sym) return JGPM_OK;
+  jgpm_ctx_s* c = g->ctx;
+  if (c->capturing) return JGPM_E_STATE;
+  int rc = ensure_sym_aux(g, false, false, st);  // COO sources
+  if (rc) return rc;
+  const int32_t nv = (int32_t)g->sym.nv;
+  const int64_t ne = g->sym.nnz;
+  int B = 1;
+  while (B < 31 && (1ll << B) < (int64_t)nv) ++B;
+  Scratch S;
+  S.st = st;
+  DevCSR r;
+  r.nv = nv;
+  r.nnz = ne;
+  r.max_deg = g->sym.max_deg;
+  if (hipMalloc(&r.rp, sizeof(int64_t) * ((size_t)nv + 1)) != hipSuccess) return JGPM_E_NOMEM;
+  if (hipMalloc(&r.ci, sizeof(int32_t) * std::max<int64_t>(ne, 1)) != hipSuccess) {
+    (void)hipFree(r.rp);
+    return JGPM_E_NOMEM;
+  }
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(r.rp);
+    (void)hipFree(r.ci);
+    return code;
+  };
+  int64_t* deg = S.get<int64_t>((size_t)nv + 1);
+  if (!deg) return fail(JGPM_E_NOMEM);
+  hipLaunchKernelGGL(k_rank_deg, dim3(grid_for(nv)), dim3(256), 0, st, g->sym.rp, g->d_ord, nv, deg);
+  hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, deg + nv, (int64_t)1, (int64_t)0);
+  if ((rc = scan_i64(S, deg, r.rp, (int64_t)nv + 1, st))) return fail(rc);
+  if (ne) {
+    uint64_t* keys = S.get<uint64_t>(ne);
+    uint64_t* alt = S.get<uint64_t>(ne);
+    if (!keys || !alt) return fail(JGPM_E_NOMEM);
+    hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(ne)), dim3(256), 0, st, g->d_src, g->sym.ci, g->d_ord, ne, B, keys);
+    hipcub::DoubleBuffer<uint64_t> db(keys, alt);
+    size_t tb = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, tb, db, (int)ne, 0, 2 * B, st) != hipSuccess) return fail(JGPM_E_HIP);
+    void* tmp = S.get<char>(tb);
+    if (!tmp) return fail(JGPM_E_NOMEM);
+    if (hipcub::DeviceRadixSort::SortKeys(tmp, tb, db, (int)ne, 0, 2 * B, st) != hipSuccess) return fail(JGPM_E_HIP);
+    hipLaunchKernelGGL(k_low_bits, dim3(grid_for(ne)), dim3(256), 0, st, db.Current(), ne, (1ull << B) - 1, r.ci);
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return fail(JGPM_E_HIP);
+  g->rsym = r;
+  g->bytes += 8 * ((int64_t)nv + 1) + 4 * ne;
+  return JGPM_OK;
+}
+
+}  // namespace jgpm

@@ -394,6 +394,7 @@ extern "C" int jgpm_graph_free(jgpm_graph_t g) {
   if (g->d_tri) (void)hipFree(g->d_tri);
   if (g->d_nsum) (void)hipFree(g->d_nsum);
   if (g->d_ord) (void)hipFree(g->d_ord);
+  free_csr(g->rsym);
   delete g;
   return JGPM_OK;
 }

@@ -71,6 +71,7 @@ struct jgpm_graph_s {
   int32_t* d_src = nullptr;                    // COO source per symmetric edge (lazy)
   int32_t* d_tri = nullptr;                    // triangles per symmetric edge (lazy)
   int64_t* d_nsum = nullptr;                   // neighbour-degree sums (lazy)
+  DevCSR rsym;                                 // symmetric CSR relabelled by d_ord (lazy, 4-cycle totals)
   int64_t bytes = 0;
   int nbatch = 0;                              // prepared batches that reference this graph
 };
@@ -83,6 +84,9 @@ int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
 // the zeroed 32-bit queue counter qctr and counters cnt[0..cnt_ints) that are
 // all zero (and are left so).
 size_t p1_counter_ints(jgpm_ctx_s* c, int32_t nv);
+// g->rsym: vertex v becomes d_ord[v] (degree order), lists re-sorted.  Built
+// once per graph (allocates; never while capturing).
+int ensure_rank_csr(jgpm_graph_s* g, hipStream_t st);
 int ensure_p1_counters(jgpm_ctx_s* c, size_t need, hipStream_t st);
 int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, int32_t* cnt, size_t cnt_ints,
               hipStream_t st);

@@ -93,19 +93,67 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
 //   sum over x < v0 of C(c_x, 2),  c_x = |{v in N(v0) : v < v0, x in N(v)}|:
 // every pair v2 < v1 of the c_x such v with the common neighbour x is one
 // term.  So a root needs one pass over its wedges (v0, v, x) with v, x < v0,
-// not an intersection per pair.  One wave per root: the first pass adds 1 to
-// c_x in the wave's dense counter array, the second takes every counter back
-// to zero with atomicExch and adds C(c, 2) for the value it took (a repeated x
-// takes 0 after its first exchange).  The wedges of 64 neighbours v at a time
-// form one flattened stream (consecutive lanes read consecutive elements).
+// not an intersection per pair.  One wave per root; the wedges of 64
+// neighbours v at a time form one flattened stream (consecutive lanes read
+// consecutive elements).  The c_x live
+//  * in the wave's LDS hash table when the root has at most P1_HS / 2
+//    wedges: open addressing on 64-bit slots (key x in the high half, count in
+//    the low half; a slot is claimed by compare-and-swap and counted by a
+//    64-bit add), summed and emptied in one sweep of the table;
+//  * otherwise in the wave's dense global counter array: a first pass adds 1
+//    to c_x, a second takes every counter back to zero with atomicExch and
+//    adds C(c, 2) for the value it took (a repeated x takes 0 after its first
+//    exchange).
+constexpr int P1_LOG_HS = 12;
+constexpr int P1_HS = 1 << P1_LOG_HS;                // LDS slots per wave
+constexpr uint64_t P1_EMPTY = 0xffffffff00000000ull;  // key -1, count 0
+
+// Calls f(x) for every wedge endpoint x of root v0 (its m0 neighbours below
+// v0 start at ci[b0]), 64 neighbours at a time.
+template <class F>
+__device__ void p1_wedges(const P1Args& a, int64_t b0, int m0, int32_t v0, int32_t* off, int64_t* lbase, F f) {
+  const int ln = lane();
+  for (int jb = 0; jb < m0; jb += 64) {
+    const int j = jb + ln;
+    int len = 0;
+    int64_t base = 0;
+    if (j < m0) {
+      const int32_t v = a.ci[b0 + j];
+      base = a.rp[v];
+      len = lb(a.ci + base, (int)(a.rp[v + 1] - base), v0);  // x < v0
+    }
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o);
+      if (ln >= o) incl += y;
+    }
+    const int T = __shfl(incl, 63);
+    off[ln] = incl - len;
+    lbase[ln] = base - (incl - len);  // element f of v's segment is ci[lbase + f]
+    if (ln == 63) off[64] = T;
+    __syncwarp();
+    int i = 0;
+    for (int fe = ln; fe < T; fe += 64) {
+      i = seg_advance(off, 64, i, fe);
+      f(a.ci[lbase[i] + fe]);
+    }
+    __syncwarp();
+  }
+}
+
 __global__ void __launch_bounds__(64 * P1_WPB) p1_wedge_kernel(P1Args a) {
+  __shared__ uint64_t s_tab[P1_WPB][P1_HS];
   __shared__ int32_t s_off[P1_WPB][65];
   __shared__ int64_t s_lb[P1_WPB][64];
   const int wv = threadIdx.x >> 6;
   const int ln = lane();
+  uint64_t* tab = s_tab[wv];
   int32_t* off = s_off[wv];
   int64_t* lbase = s_lb[wv];
   int32_t* cnt = a.cnt + ((int64_t)blockIdx.x * P1_WPB + wv) * a.nv;
+  for (int k = ln; k < P1_HS; k += 64) tab[k] = P1_EMPTY;
+  __syncwarp();
   uint64_t acc = 0;
   for (;;) {
     int64_t t = 0;
@@ -115,41 +163,53 @@ __global__ void __launch_bounds__(64 * P1_WPB) p1_wedge_kernel(P1Args a) {
     const int32_t v0 = (int32_t)r;
     const int64_t b0 = a.rp[v0];
     const int m0 = lb(a.ci + b0, (int)(a.rp[v0 + 1] - b0), v0);  // neighbours below v0
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int jb = 0; jb < m0; jb += 64) {
-        const int j = jb + ln;
-        int len = 0;
-        int64_t base = 0;
-        if (j < m0) {
-          const int32_t v = a.ci[b0 + j];
-          base = a.rp[v];
-          len = lb(a.ci + base, (int)(a.rp[v + 1] - base), v0);  // x < v0
-        }
-        int incl = len;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const int y = __shfl_up(incl, o);
-          if (ln >= o) incl += y;
-        }
-        const int T = __shfl(incl, 63);
-        off[ln] = incl - len;
-        lbase[ln] = base - (incl - len);  // element f of v's segment is ci[lbase + f]
-        if (ln == 63) off[64] = T;
-        __syncwarp();
-        int i = 0;
-        for (int f = ln; f < T; f += 64) {
-          i = seg_advance(off, 64, i, f);
-          const int32_t x = a.ci[lbase[i] + f];
-          if (pass == 0) {
-            atomicAdd(&cnt[x], 1);
-          } else {
-            const uint64_t c = (uint32_t)atomicExch(&cnt[x], 0);
-            acc += c * (c - 1) / 2;
+    // wedge count of the root, to pick the table
+    int64_t w = 0;
+    for (int j = ln; j < m0; j += 64) {
+      const int32_t v = a.ci[b0 + j];
+      const int64_t base = a.rp[v];
+      w += lb(a.ci + base, (int)(a.rp[v + 1] - base), v0);
+    }
+    w = wsum64(w);
+    if (w <= P1_HS / 2) {
+      p1_wedges(a, b0, m0, v0, off, lbase, [&](int32_t x) {
+        const uint64_t key = (uint64_t)(uint32_t)x << 32;
+        int h = (int)(((uint32_t)x * 2654435761u) >> (32 - P1_LOG_HS));
+        for (;;) {
+          uint64_t cur = tab[h];
+          if ((cur & P1_EMPTY) == key) {
+            atomicAdd((unsigned long long*)&tab[h], 1ull);
+            break;
           }
+          if (cur == P1_EMPTY) {
+            cur = atomicCAS((unsigned long long*)&tab[h], P1_EMPTY, key | 1ull);
+            if (cur == P1_EMPTY) break;
+            if ((cur & P1_EMPTY) == key) {  // claimed by another lane for the same x
+              atomicAdd((unsigned long long*)&tab[h], 1ull);
+              break;
+            }
+          }
+          h = (h + 1) & (P1_HS - 1);
         }
-        __syncwarp();
+      });
+      __syncwarp();
+      for (int k = ln; k < P1_HS; k += 64) {
+        const uint64_t v = tab[k];
+        if (v != P1_EMPTY) {
+          const uint64_t c = (uint32_t)v;
+          acc += c * (c - 1) / 2;
+          tab[k] = P1_EMPTY;
+        }
       }
+      __syncwarp();
+    } else {
+      p1_wedges(a, b0, m0, v0, off, lbase, [&](int32_t x) { atomicAdd(&cnt[x], 1); });
       __threadfence();  // the increments are performed before they are taken back
+      p1_wedges(a, b0, m0, v0, off, lbase, [&](int32_t x) {
+        const uint64_t c = (uint32_t)atomicExch(&cnt[x], 0);
+        acc += c * (c - 1) / 2;
+      });
+      __threadfence();
     }
   }
   acc = wave_sum_u64(acc);
