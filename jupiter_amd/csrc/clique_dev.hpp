@@ -773,6 +773,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     }
   }
   if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (G > 1 && r == 0) s_wtot[G] = 0;  // row counter of the count phase (multi-word rows)
   Grp<G>::sync();
   if constexpr (!LROWS && NEED_ROWS) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }  // build
@@ -790,7 +791,21 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     const RowT R{rowp, lay};
     int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;
     uint64_t* Rc = reinterpret_cast<uint64_t*>(U + LY::C_SC) + (size_t)wv * 64;
-    for (int i = wv; i < d; i += G) {
+    // waves take rows in order from a shared counter (s_wtot[G], zeroed
+    // before the group's last barrier): early rows in DAG order hold the most
+    // work, so a fixed interleave leaves waves idle at the end.  4-cliques
+    // on the narrower tiers have too little work per row to pay for it.
+    constexpr bool DYN = G > 1 && (K >= 5 || WMAX >= 16);
+    auto next_row = [&](int cur) -> int {
+      if constexpr (!DYN) {
+        return cur + G;
+      } else {
+        int t = 0;
+        if (ln == 0) t = atomicAdd(&s_wtot[G], 1);
+        return __shfl(t, 0);
+      }
+    };
+    for (int i = DYN ? next_row(0) : wv; i < d; i = next_row(i)) {
       // population and nonzero word window of row i (W <= 32 words)
       const uint64_t myword = ln < W ? R(i, ln) : 0ull;
       const uint64_t nz = __ballot(myword != 0ull);

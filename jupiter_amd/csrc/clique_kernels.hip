@@ -152,6 +152,7 @@ __global__ void __launch_bounds__(64 * 8) clique_count_kernel(CliqueArgs a) {
     const int n = d * RowLayout((d + 63) >> 6).S;
     const uint64_t* src = a.gscr + (size_t)task * ROWW;
     for (int t = threadIdx.x; t < n; t += 64 * G) rowp[t] = __builtin_nontemporal_load(src + t);
+    if (threadIdx.x == 0) s_wtot[G] = 0;
     __syncthreads();
     process_root<G, SPLIT_WMAX, K, false, true, 2>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena, rowp, s_wtot, acc);
   }
