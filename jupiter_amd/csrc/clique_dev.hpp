@@ -555,7 +555,7 @@ __device__ __forceinline__ int hfind16(const int16_t* h16, const int32_t* L, int
 // LDS layout of one root slot.  The build-phase arrays (segment offsets,
 // list bases, long-list degrees) and the count-phase arrays (compacted row
 // lists, deferred rows, the compressed bitmap) share one region.
-template <int G, int WMAX, int K, bool BYTES, bool LROWS, bool CNT = true>
+template <int G, int WMAX, int K, bool BYTES, bool LROWS, bool CNT = true, bool BUILD = true>
 struct Lay {
   static constexpr int DM = 64 * WMAX;
   static constexpr bool ROWS = K >= 4;
@@ -565,7 +565,7 @@ struct Lay {
   static constexpr int CWMAX = NC > 0 ? NC / 64 : 1;
   static constexpr size_t al(size_t x) { return (x + 15) & ~size_t(15); }
   static constexpr size_t L_OFF = 0;
-  static constexpr size_t W_OFF = al(L_OFF + 4 * DM);
+  static constexpr size_t W_OFF = al(L_OFF + (BUILD ? 4 * DM : 0));  // the root list (build only)
   static constexpr size_t R_OFF = al(W_OFF + (BYTES ? 4 * DM : 0));
   static constexpr size_t U_OFF = al(R_OFF + ((LROWS && ROWS) ? (size_t)8 * DM * WMAX : 0));
   // build view
@@ -583,9 +583,7 @@ struct Lay {
   static constexpr size_t B_END = al(B_RNK + (MULTI ? 2 * DM : 0));
   // count view
   static constexpr size_t C_J = 0;                                          // int16 [G][DM]
-  static constexpr size_t C_JB = al(C_J + (MULTI ? 2 * G * DM : 0));        // (unused)
-  static constexpr size_t C_DEF = C_JB;
-  static constexpr size_t C_COMP = al(C_DEF + (MULTI ? 2 * DM : 0));        // u64 [NC * CWMAX]
+  static constexpr size_t C_COMP = al(C_J + (MULTI ? 2 * G * DM : 0));     // u64 [NC * CWMAX]
   static constexpr size_t C_SC = al(C_COMP + (size_t)8 * NC * CWMAX);       // u64 [G][64]
   static constexpr size_t C_SCW = al(C_SC + (MULTI ? (size_t)8 * G * 64 : 0));  // u32 [G][64]
   static constexpr size_t C_WCB = C_SCW;
@@ -605,7 +603,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   // PHASE 0: build and count; 1: build only (rows left in rowp); 2: count
   // only (rowp already holds the rows of this root)
   static_assert(PHASE == 0 || (!BYTES && K >= 4), "split phases are for counting with rows");
-  using LY = Lay<G, WMAX, K, BYTES, LROWS, PHASE != 1>;
+  using LY = Lay<G, WMAX, K, BYTES, LROWS, PHASE != 1, PHASE != 2>;
   constexpr int DM = LY::DM;
   constexpr bool NEED_ROWS = LY::ROWS;
   const int r = Grp<G>::rank();

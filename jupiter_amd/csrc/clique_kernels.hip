@@ -130,10 +130,17 @@ __global__ void __launch_bounds__(64 * 16) clique_build_kernel(CliqueArgs a) {
   }
 }
 
+// Waves per root of the count kernel: 12 (three per SIMD, all the LDS
+// allows beside the rows) where the count fits the registers of three waves;
+// the 5- and 7-clique counts hold 2x2 tiles of matrix-core accumulators and
+// would spill there, so they keep 8.
 template <int K>
-__global__ void __launch_bounds__(64 * 8) clique_count_kernel(CliqueArgs a) {
-  constexpr int G = 8;
-  using LY = Lay<G, SPLIT_WMAX, K, false, true, true>;
+constexpr int split_count_waves() { return (K == 5 || K == 7) ? 8 : 12; }
+
+template <int K>
+__global__ void __launch_bounds__(64 * split_count_waves<K>()) clique_count_kernel(CliqueArgs a) {
+  constexpr int G = split_count_waves<K>();
+  using LY = Lay<G, SPLIT_WMAX, K, false, true, true, false>;
   __shared__ __attribute__((aligned(16))) unsigned char s_arena[LY::SLOT];
   __shared__ int32_t s_wtot[G + 1];
   __shared__ int64_t s_task;
@@ -162,7 +169,7 @@ __global__ void __launch_bounds__(64 * 8) clique_count_kernel(CliqueArgs a) {
 template <int K>
 static hipError_t launch_split_k(bool build, const CliqueArgs& a, int grid, hipStream_t st) {
   if (build) hipLaunchKernelGGL(clique_build_kernel<K>, dim3(grid), dim3(64 * 16), 0, st, a);
-  else hipLaunchKernelGGL(clique_count_kernel<K>, dim3(grid), dim3(64 * 8), 0, st, a);
+  else hipLaunchKernelGGL(clique_count_kernel<K>, dim3(grid), dim3(64 * split_count_waves<K>()), 0, st, a);
   return hipGetLastError();
 }
 
