@@ -794,6 +794,11 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     // work, so a fixed interleave leaves waves idle at the end.  4-cliques
     // on the narrower tiers have too little work per row to pay for it.
     constexpr bool DYN = G > 1 && (K >= 5 || WMAX >= 16);
+    // 5-cliques of every row (also |S| <= 64) on the matrix cores from 256
+    // wide up: the successor bitmap below leaves lanes idle on rows of few
+    // members (measured 79.8 -> 75.3 ms for one P14 query on R-MAT scale
+    // 22); the 128-wide tier is faster without (6.4 vs 8.2 ms)
+    constexpr bool MMA5 = K == 5 && !BYTES && WMAX >= 4;
     auto next_row = [&](int cur) -> int {
       if constexpr (!DYN) {
         return cur + G;
@@ -832,7 +837,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
             acc += c;
           }
         }
-      } else if (n <= 64) {
+      } else if (!MMA5 && n <= 64) {
         // compressed |S| x |S| successor bitmap, one word per member
         uint64_t myrow = 0;
         const int js = ln < n ? J[ln] : 0;
@@ -856,7 +861,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
           else acc += rec1<K - 3, false>(myrow, Rc, wL);
         }
       } else {
-        if constexpr (K == 5 && !BYTES && WMAX >= 4) {
+        if constexpr (MMA5) {
           // tiles of block pairs pay for their registers only on the widest roots
           if constexpr (WMAX >= 16) acc += tri_mfma(R, J, n, i, whi, d - 1);
           else acc += tri_mfma1(R, J, n, i, whi, d - 1);
