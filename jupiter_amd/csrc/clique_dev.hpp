@@ -208,12 +208,25 @@ struct RowLayout {
   __device__ int operator()(int i, int w) const { return i * S + (w ^ ((i >> sh) & X)); }
 };
 // Row accessors: word w of row i, and its 32-bit half h (bits 32h .. 32h + 31).
+// A cursor fixes row i and half h: its byte base and the row's word swizzle
+// pre-scaled to bytes, so a read at word w costs one xor-add
+// (8 (w ^ s) = 8w ^ 8s).
+struct RowCur {
+  const unsigned char* b;
+  uint32_t sw;
+};
 struct RowsLds {
   uint64_t* p;
   RowLayout lay;
   __device__ uint64_t operator()(int i, int w) const { return p[lay(i, w)]; }
   __device__ uint32_t half(int i, int w, int h) const {
     return reinterpret_cast<const uint32_t*>(p)[2 * lay(i, w) + h];
+  }
+  __device__ RowCur cur(int i, int h) const {
+    return {reinterpret_cast<const unsigned char*>(p + (size_t)i * lay.S) + 4 * h, 8u * (uint32_t)((i >> lay.sh) & lay.X)};
+  }
+  __device__ uint32_t at(const RowCur& c, int w) const {
+    return *reinterpret_cast<const uint32_t*>(c.b + ((8u * (uint32_t)w) ^ c.sw));
   }
 };
 struct RowsGlobal {
@@ -222,6 +235,12 @@ struct RowsGlobal {
   __device__ uint64_t operator()(int i, int w) const { return __builtin_nontemporal_load(p + lay(i, w)); }
   __device__ uint32_t half(int i, int w, int h) const {
     return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p) + 2 * lay(i, w) + h);
+  }
+  __device__ RowCur cur(int i, int h) const {
+    return {reinterpret_cast<const unsigned char*>(p + (size_t)i * lay.S) + 4 * h, 8u * (uint32_t)((i >> lay.sh) & lay.X)};
+  }
+  __device__ uint32_t at(const RowCur& c, int w) const {
+    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(c.b + ((8u * (uint32_t)w) ^ c.sw)));
   }
 };
 
@@ -461,17 +480,24 @@ __device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, i
       const int x = bx < n ? (int)J[bx] : -1;
       const int rx = x >= 0 ? x : zr;
       // word q + 1's row halves are loaded before word q's table reads
-      uint32_t wi = R.half(i, q0, half), wy = R.half(ry, q0, half), wx = R.half(rx, q0, half);
-      auto step = [&](int q, const v16f32& c) -> v16f32 {
-        const uint32_t sa = wy & wi, sb = wx;
+      // (two register sets alternate, so the loop moves no values)
+      const RowCur ki = R.cur(i, half), ky = R.cur(ry, half), kx = R.cur(rx, half);
+      uint32_t va[3] = {R.at(ki, q0), R.at(ky, q0), R.at(kx, q0)}, vb[3];
+      auto step = [&](int q, const v16f32& c, const uint32_t(&v)[3], uint32_t(&nx)[3]) -> v16f32 {
+        const uint32_t sa = v[1] & v[0], sb = v[2];
         const int wn = min(q + 1, whi);
-        wi = R.half(i, wn, half);
-        wy = R.half(ry, wn, half);
-        wx = R.half(rx, wn, half);
+        nx[0] = R.at(ki, wn);
+        nx[1] = R.at(ky, wn);
+        nx[2] = R.at(kx, wn);
         return mma_fp4(enc_a(sa), enc_b(sb), c);
       };
-      v16f32 acc = step(q0, zero16());
-      for (int q = q0 + 1; q <= whi; ++q) acc = step(q, acc);
+      v16f32 acc = step(q0, zero16(), va, vb);
+      int q = q0 + 1;
+      for (; q + 1 <= whi; q += 2) {
+        acc = step(q, acc, vb, va);
+        acc = step(q + 1, acc, va, vb);
+      }
+      if (q <= whi) acc = step(q, acc, vb, va);
       s += edge_masked_sum(R, J, n, whi, Y, x, half, acc);
     }
   }
@@ -506,16 +532,18 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, in
       const int rx0 = x0 >= 0 ? x0 : zr, rx1 = x1 >= 0 ? x1 : zr;
       v16f32 c00, c01, c10, c11, m00, m01, m10, m11;
       // word q + 1's row halves are loaded before word q's table reads
-      uint32_t wi = R.half(i, q0, half), wy0 = R.half(ry0, q0, half), wy1 = R.half(ry1, q0, half);
-      uint32_t wx0 = R.half(rx0, q0, half), wx1 = R.half(rx1, q0, half);
-      auto step = [&](int q, bool first, bool with_mask) {
-        const uint32_t sa0 = wy0 & wi, sa1 = wy1 & wi, sb0 = wx0, sb1 = wx1;
+      // (two register sets alternate, so the loop moves no values)
+      const RowCur ki = R.cur(i, half), ky0 = R.cur(ry0, half), ky1 = R.cur(ry1, half);
+      const RowCur kx0 = R.cur(rx0, half), kx1 = R.cur(rx1, half);
+      uint32_t va[5] = {R.at(ki, q0), R.at(ky0, q0), R.at(ky1, q0), R.at(kx0, q0), R.at(kx1, q0)}, vb[5];
+      auto step = [&](int q, bool first, bool with_mask, const uint32_t(&v)[5], uint32_t(&nx)[5]) {
+        const uint32_t sa0 = v[1] & v[0], sa1 = v[2] & v[0], sb0 = v[3], sb1 = v[4];
         const int wn = min(q + 1, whi);
-        wi = R.half(i, wn, half);
-        wy0 = R.half(ry0, wn, half);
-        wy1 = R.half(ry1, wn, half);
-        wx0 = R.half(rx0, wn, half);
-        wx1 = R.half(rx1, wn, half);
+        nx[0] = R.at(ki, wn);
+        nx[1] = R.at(ky0, wn);
+        nx[2] = R.at(ky1, wn);
+        nx[3] = R.at(kx0, wn);
+        nx[4] = R.at(kx1, wn);
         const v8i32 a0 = enc_a(sa0), a1 = enc_a(sa1);
         const v8i32 b0 = enc_b(sb0), b1 = enc_b(sb1);
         const v16f32 z = zero16();
@@ -531,8 +559,13 @@ __device__ uint64_t tri_mfma(const RowT& R, const int16_t* J, int n_, int i_, in
           m11 = mma_fp4(e1, b1, first ? z : m11);
         }
       };
-      step(q0, true, true);  // word q0 holds the first Y member
-      for (int q = q0 + 1; q <= whi; ++q) step(q, false, q <= qm);
+      step(q0, true, true, va, vb);  // word q0 holds the first Y member
+      int q = q0 + 1;
+      for (; q + 1 <= whi; q += 2) {
+        step(q, false, q <= qm, vb, va);
+        step(q + 1, false, q + 1 <= qm, va, vb);
+      }
+      if (q <= whi) step(q, false, q <= qm, vb, va);
       s += (uint64_t)dot16(c00, m00) + dot16(c01, m01) + dot16(c10, m10) + dot16(c11, m11);
     }
   }
