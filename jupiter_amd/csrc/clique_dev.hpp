@@ -780,13 +780,19 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       }
     }
   }
-  // ---- long lists: binary search every element of L in them
+  // ---- long lists: binary search every element of L in them.  The
+  // (list, 64-member chunk) pairs are dealt to the waves round-robin, so a
+  // root with fewer members than 64 G still keeps every wave busy.
+  const int nch = (d + 63) >> 6;
+  int kk = 0;  // pairs dealt so far (the same on every wave)
   for (int i = 0; i < d; ++i) {
     if (!((hfl[i >> 5] >> (i & 31)) & 1u)) continue;
     const int du = (int)(rp[L[i] + 1] - ub[i]);
     const int32_t* lst = ci + ub[i];
     const int ri = RANK ? (int)rnk[i] : i;
-    for (int t0 = wv * 64; t0 < d; t0 += Grp<G>::size) {
+    const int c0 = ((wv - kk) % G + G) % G;  // first chunk c with (kk + c) % G == wv
+    kk += nch;
+    for (int t0 = c0 * 64; t0 < d; t0 += Grp<G>::size) {
       int t = t0 + ln;
       bool f = t < d && bsearch_global(lst, du, L[t]);
       if constexpr (K == 3) {
