@@ -143,7 +143,7 @@ typedef struct {
   int32_t small_max_deg;    /* roots with out-degree <= this use one wave each (<= 64)   */
   int32_t lds_max_deg;      /* roots up to this out-degree keep their bitmap in LDS      */
   int32_t gallop_ratio;     /* neighbour list longer than ratio*d(root) -> binary search
-                               (ratio/4 for roots of out-degree <= 256)                     */
+                               (ratio/4 for roots of out-degree <= 128, ratio/8 <= 256)     */
   int32_t waves_per_block;  /* waves per workgroup for the wave-per-root kernel           */
   int32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * num_cu              */
   /* batched launches (jgpm_count_batch, jgpm_batch_*, serving) */

@@ -453,10 +453,12 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
     a.ci = g->dag.ci;
     a.ord = g->d_ord;
     // roots up to 256 wide search all members of a long list in one pass of
-    // their waves, so binary search pays from a quarter of the ratio (R-MAT
-    // 22, one quarter of the roots: 64-wide tier 3.65 -> 3.1 ms, 128-wide
-    // 5.2 -> 4.6 ms for P13; the wider tiers measured no gain)
-    a.ratio = std::max(1, tier <= jk::TIER_LDS4 ? c->tune.gallop_ratio / 4 : c->tune.gallop_ratio);
+    // their waves, so binary search pays earlier: from a quarter of the ratio
+    // up to 128 wide, an eighth at 256 (R-MAT 22, one quarter of the roots:
+    // 64-wide tier 3.65 -> 3.1 ms, 128-wide 5.2 -> 4.6 ms, 256-wide 8.4 ->
+    // 8.2 ms for P13; the wider tiers measured no gain)
+    const int gr = c->tune.gallop_ratio;
+    a.ratio = std::max(1, tier == jk::TIER_LDS4 ? gr / 8 : tier < jk::TIER_LDS4 ? gr / 4 : gr);
     a.qctr = qctr_base ? qctr_base + tier : c->d_q + (out_slot * jk::NUM_TIERS + tier) % CTX_SLOTS;
     a.out = out_ptr ? out_ptr : c->d_out + out_slot + (split_slots ? tier : 0);
     a.gscr = nullptr;
