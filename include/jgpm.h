@@ -140,11 +140,15 @@ int  jgpm_work_bytes_split(jgpm_graph_t g, int pattern, int64_t root_begin, int6
  * knob (clamped to its range), 0 keeps the current value, < 0 restores the
  * default. */
 typedef struct {
-  int32_t small_max_deg;    /* roots with out-degree <= this use one wave each (<= 64)   */
-  int32_t lds_max_deg;      /* roots up to this out-degree keep their bitmap in LDS      */
+  /* small_max_deg, lds_max_deg and waves_per_block are kept and reported but
+   * not read by the kernels: the clique tiers are compiled for out-degrees
+   * 64 (one wave per root, 4 roots per block), 128, 256, 512, 1024 (bitmap
+   * rows in LDS) and 2048 (rows in global scratch). */
+  int32_t small_max_deg;    /* (fixed: 64) roots that use one wave each                  */
+  int32_t lds_max_deg;      /* (fixed: 1024) roots that keep their bitmap in LDS          */
   int32_t gallop_ratio;     /* neighbour list longer than ratio*d(root) -> binary search
                                (ratio/4 for roots of out-degree <= 128, ratio/8 <= 256)     */
-  int32_t waves_per_block;  /* waves per workgroup for the wave-per-root kernel           */
+  int32_t waves_per_block;  /* (fixed: 4) waves per workgroup of the wave-per-root kernel  */
   int32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * num_cu              */
   /* batched launches (jgpm_count_batch, jgpm_batch_*, serving) */
   int32_t batch_hub_deg;    /* clique roots above this out-degree leave the wave-per-root
