@@ -229,7 +229,7 @@ static int graph_from_host(jgpm_ctx_s* c, const jgpm_csr_s* sym, const jgpm_csr_
 static jgpm_tuning default_tuning() {
   jgpm_tuning t{};
   t.small_max_deg = 64;
-  t.lds_max_deg = 512;
+  t.lds_max_deg = 1024;  // reported only: the tier widths are compiled in
   t.gallop_ratio = 32;
   t.waves_per_block = 4;
   t.blocks_per_cu = 8;
