@@ -504,6 +504,61 @@ __device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, i
   return s;
 }
 
+// 4-cliques of one root on the matrix cores: the sum over DAG edges x -> y
+// inside L of |R[x] & R[y]| (each triangle x < y < z of L counted at its
+// first edge): the block-pair form of tri_mfma1 with every member of L in
+// the blocks, so the members are the row positions themselves.  Waves take
+// the Y blocks wv, wv + G, ...; block Y's positions lie in word Y / 2, so
+// the edge mask is one word of row x per block pair.
+template <class RowT>
+__device__ uint64_t tri4_mfma(const RowT& R, int d_, int W_, int wv_, int G) {
+  const int d = uni(d_), whi = uni(W_) - 1, wv = uni(wv_), zr = d - 1;
+  const int ln = lane_id();
+  const int col = ln & 31, half = ln >> 5;
+  const int T = (d + 31) >> 5;
+  uint64_t s = 0;
+  for (int Y = wv; Y < T; Y += G) {
+    const int q0 = (32 * Y + 1) >> 6;  // R[y] is empty up to the block's first member
+    const int ay = 32 * Y + col;
+    const int ry = ay < d ? ay : zr;
+    for (int X = 0; X <= Y; ++X) {
+      const int bx = 32 * X + col;
+      const int rx = bx < d ? bx : zr;
+      const RowCur ky = R.cur(ry, half), kx = R.cur(rx, half);
+      uint32_t va[2] = {R.at(ky, q0), R.at(kx, q0)}, vb[2];
+      auto step = [&](int q, const v16f32& c, const uint32_t(&v)[2], uint32_t(&nx)[2]) -> v16f32 {
+        const uint32_t sa = v[0], sb = v[1];
+        const int wn = min(q + 1, whi);
+        nx[0] = R.at(ky, wn);
+        nx[1] = R.at(kx, wn);
+        return mma_fp4(enc_a(sa), enc_b(sb), c);
+      };
+      v16f32 acc = step(q0, zero16(), va, vb);
+      int q = q0 + 1;
+      for (; q + 1 <= whi; q += 2) {
+        acc = step(q, acc, vb, va);
+        acc = step(q + 1, acc, va, vb);
+      }
+      if (q <= whi) acc = step(q, acc, vb, va);
+      // edge x -> y: bit y of R[x]; accumulator 4g + e holds row 8g + 4 half + e
+      float f = 0.f;
+      if (bx < d) {
+        const uint64_t rw = R(bx, Y >> 1);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int yy = 32 * Y + 8 * g + 4 * half + e;
+            if (yy < d && ((rw >> (yy & 63)) & 1ull)) f += acc[4 * g + e];
+          }
+        }
+      }
+      s += (uint32_t)f;
+    }
+  }
+  return s;
+}
+
 // The same count with 2 x 2 tiles of block pairs (widest roots): each word's
 // four operand slices feed four independent count accumulators.  The edge
 // mask is the product E_Y R_X^T of one-hot rows at the Y members' own
@@ -827,6 +882,13 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   } else {
     using RowT = typename std::conditional<LROWS, RowsLds, RowsGlobal>::type;
     const RowT R{rowp, lay};
+    // 4-cliques of the 1024-wide and wider roots as one masked block product
+    // over the whole root (the split count 3.2 -> 0.75 ms for P13 on R-MAT 22,
+    // one quarter of the roots); the fused narrower tiers lose more to the
+    // registers it costs their build than they gain (10.8 -> 13.0 ms at 512)
+    if constexpr (K == 4 && !BYTES && WMAX >= 16) {
+      acc += tri4_mfma(R, d, W, wv, G);
+    } else {
     int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;
     uint64_t* Rc = reinterpret_cast<uint64_t*>(U + LY::C_SC) + (size_t)wv * 64;
     // waves take rows in order from a shared counter (s_wtot[G], zeroed
@@ -981,6 +1043,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         }
       }
       __syncwarp();
+    }
     }
   }
   Grp<G>::sync();
