@@ -882,11 +882,12 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   } else {
     using RowT = typename std::conditional<LROWS, RowsLds, RowsGlobal>::type;
     const RowT R{rowp, lay};
-    // 4-cliques of the 1024-wide and wider roots as one masked block product
-    // over the whole root (the split count 3.2 -> 0.75 ms for P13 on R-MAT 22,
-    // one quarter of the roots); the fused narrower tiers lose more to the
-    // registers it costs their build than they gain (10.8 -> 13.0 ms at 512)
-    if constexpr (K == 4 && !BYTES && WMAX >= 16) {
+    // 4-cliques of the 512-wide and wider roots as one masked block product
+    // over the whole root (P13 on R-MAT 22, one quarter of the roots: split
+    // count 3.2 -> 0.75 ms; 512-wide tier 10.8 -> 9.7 ms with its registers
+    // held to the 6 waves per SIMD its LDS allows, 13.0 ms without); the
+    // 256-wide and narrower tiers keep the bitwise count
+    if constexpr (K == 4 && !BYTES && WMAX >= 8) {
       acc += tri4_mfma(R, d, W, wv, G);
     } else {
     int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;

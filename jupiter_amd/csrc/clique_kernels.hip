@@ -42,12 +42,14 @@ namespace jk {
 // Waves per SIMD the kernel's registers are held to.  These kernels are
 // bound by latency rather than by any one unit, so on the tiers below the
 // registers are capped at the occupancy their LDS allows (measured faster on
-// the 5-clique tiers, also where the 4-word tier spills 20 bytes a lane);
+// the 5-clique tiers and the 4-clique 512-wide tier, also where they spill a
+// few bytes a lane);
 // elsewhere the compiler's own choice stands (already at the LDS or hardware
 // limit, or the cap would spill).
 template <int WMAX, int K, bool BYTES>
 constexpr int min_waves() {
   if (BYTES) return 1;
+  if (K == 4 && WMAX == 8) return 6;
   if (K == 5 && WMAX == 4) return 8;
   if (K == 5 && WMAX == 8) return 6;
   if (K == 6 && WMAX == 4) return 7;
