@@ -118,7 +118,10 @@ int  jgpm_graph_bytes(jgpm_graph_t g, int64_t* bytes);
  * source vertices v0 of the matched edges, as the reference's per-GPU edge
  * ranges (src/jupiter_gpm.cu:153-162,373-375).  root_end = -1 means nv.
  * Blocks until the count is on the host.  kernel_ms (may be NULL) receives
- * the device time between the first and last kernel of the query. */
+ * the device time between the first and last kernel of the query.
+ * JGPM_I_C4, JGPM_I_DIAMOND, JGPM_I_TAILTRI and JGPM_I_STAR3 combine counts
+ * that credit an occurrence to different vertices, so they are defined on the
+ * whole graph only: any other range returns JGPM_E_INVALID. */
 int  jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
                 uint64_t* count, double* kernel_ms);
 /* Algorithmic bytes of the reference's canonical loop nest for the same

@@ -33,9 +33,20 @@ class Pattern {
   Pattern& add_edge(vidType u, vidType v);
   Pattern& remove_edge(vidType u, vidType v);
   std::string to_string() const;
+  // name tests of the reference (include/pattern.hh:55-61); set_name gives
+  // the reference's names (common/pattern.cc:3-27), so is_rectangle,
+  // is_pentagon, is_house and is_4color_square stay false there as here
+  bool is_wedge() const { return name_ == "wedge"; }
   bool is_triangle() const { return name_ == "triangle"; }
   bool is_diamond() const { return name_ == "diamond"; }
   bool is_rectangle() const { return name_ == "rectangle"; }
+  bool is_pentagon() const { return name_ == "pentagon"; }
+  bool is_house() const { return name_ == "house"; }
+  bool is_4color_square() const { return name_ == "4color-square"; }
+  vidType get_neighbor(vidType v, vidType i) const { return adj_list.at(v)[i]; }
+  // pattern vertex ids, ascending
+  std::vector<vidType> v_list() const;
+  void print_meta_data() const;
   int pattern_id() const;
 
  private:

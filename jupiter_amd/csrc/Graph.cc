@@ -56,7 +56,70 @@ void Graph::assign(std::vector<eidType> rowptr, std::vector<vidType> colidx) {
   if (max_degree > VertexSet::MAX_DEGREE) VertexSet::MAX_DEGREE = max_degree;
 }
 
-VertexSet Graph::N(vidType v) { return VertexSet(edges.data() + vertices[v], get_degree(v), v); }
+VertexSet Graph::N(vidType v) const { return VertexSet(adj_ptr(v), get_degree(v), v); }
+
+bool Graph::is_connected(std::vector<vidType> sg) const {
+  if (sg.empty()) return false;
+  std::sort(sg.begin(), sg.end());
+  sg.erase(std::unique(sg.begin(), sg.end()), sg.end());
+  std::vector<char> seen(sg.size(), 0);
+  std::vector<size_t> stack{0};
+  seen[0] = 1;
+  size_t reached = 1;
+  while (!stack.empty()) {
+    size_t i = stack.back();
+    stack.pop_back();
+    for (size_t j = 0; j < sg.size(); ++j)
+      if (!seen[j] && is_connected(sg[i], sg[j])) {
+        seen[j] = 1;
+        ++reached;
+        stack.push_back(j);
+      }
+  }
+  return reached == sg.size();
+}
+
+eidType Graph::init_edgelist(bool sym_break, bool) {
+  if (nnz != 0) return nnz;
+  nnz = sym_break ? n_edges / 2 : n_edges;
+  src_list.assign((size_t)n_edges, 0);
+  for (vidType v = 0; v < n_vertices; ++v)
+    for (eidType e = vertices[v]; e < vertices[v + 1]; ++e) src_list[e] = v;
+  return nnz;
+}
+
+void Graph::sort_graph() {
+  // stable counting sort of the vertices by degree: order[k] is the vertex
+  // that gets the new id k
+  std::vector<eidType> cnt((size_t)max_degree + 2, 0);
+  for (vidType v = 0; v < n_vertices; ++v) ++cnt[get_degree(v) + 1];
+  for (size_t d = 1; d < cnt.size(); ++d) cnt[d] += cnt[d - 1];
+  std::vector<vidType> order((size_t)n_vertices), rank((size_t)n_vertices);
+  for (vidType v = 0; v < n_vertices; ++v) order[cnt[get_degree(v)]++] = v;
+  for (vidType k = 0; k < n_vertices; ++k) rank[order[k]] = k;
+  std::vector<eidType> rp((size_t)n_vertices + 1, 0);
+  std::vector<vidType> ci((size_t)n_edges);
+  for (vidType k = 0; k < n_vertices; ++k) rp[k + 1] = rp[k] + get_degree(order[k]);
+  for (vidType k = 0; k < n_vertices; ++k) {
+    const vidType v = order[k];
+    eidType o = rp[k];
+    for (eidType e = vertices[v]; e < vertices[v + 1]; ++e) ci[o++] = rank[edges[e]];
+    std::sort(ci.begin() + rp[k], ci.begin() + rp[k + 1]);
+  }
+  vertices = std::move(rp);
+  edges = std::move(ci);
+  nnz = 0;
+  src_list.clear();
+}
+
+void Graph::print_graph() const {
+  std::cout << "Printing the graph: \n";
+  for (vidType v = 0; v < n_vertices; ++v) {
+    std::cout << "vertex " << v << ": degree = " << get_degree(v) << " edgelist = [ ";
+    for (eidType e = edge_begin(v); e != edge_end(v); ++e) std::cout << getEdgeDst(e) << " ";
+    std::cout << "]\n";
+  }
+}
 
 bool Graph::is_connected(vidType v, vidType u) const {
   if (get_degree(v) > get_degree(u)) std::swap(u, v);
@@ -78,6 +141,8 @@ void Graph::orientation() {
   if (st) throw std::runtime_error(std::string("orientation: ") + jgpm_strerror(st));
   take(dag, vertices, edges, n_vertices, n_edges, max_degree);
   is_dag_ = true;
+  nnz = 0;
+  src_list.clear();
 }
 
 void Graph::dump_binary(std::string prefix) {

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <fstream>
 #include <functional>
+#include <iostream>
 #include <iterator>
 #include <numeric>
 #include <sstream>
@@ -64,19 +65,30 @@ bool Pattern::is_connected(vidType u, vidType v) const {
   return std::find(it->second.begin(), it->second.end(), v) != it->second.end();
 }
 
+// Names by vertex and edge count only, as the reference does: every
+// 3-edge 4-vertex pattern is "3-star" and every 4-edge one "square"
+// (common/pattern.cc:3-27); pattern_id() tells the shapes apart.
 void Pattern::set_name() {
   const int n = num_vertices, m = num_edges;
   name_ = "unknown";
   if (n == 3) name_ = m == 2 ? "wedge" : "triangle";
   if (n == 4) {
-    std::vector<int> deg;
-    for (auto& kv : adj_list) deg.push_back((int)kv.second.size());
-    std::sort(deg.begin(), deg.end());
-    if (m == 3) name_ = deg[3] == 3 ? "3-star" : "4-path";
-    if (m == 4) name_ = deg[3] == 2 ? "rectangle" : "tailed_triangle";
+    if (m == 3) name_ = "3-star";
+    if (m == 4) name_ = "square";
     if (m == 5) name_ = "diamond";
     if (m == 6) name_ = "4-clique";
   }
+}
+
+std::vector<vidType> Pattern::v_list() const {
+  std::vector<vidType> vs;
+  for (auto& kv : adj_list) vs.push_back(kv.first);  // std::map: ascending
+  return vs;
+}
+
+void Pattern::print_meta_data() const {
+  std::cout << "|V|: " << num_vertices << ", |E|: " << num_edges << ", |\u03A3|: 0" << std::endl;
+  std::cout << "Max Degree: " << max_degree << ", Max Label Frequency: 0" << std::endl;
 }
 
 std::string Pattern::to_string() const {

@@ -403,7 +403,8 @@ __device__ __forceinline__ v16f32 zero16() {
 // the entries of members ym = 32Y + 8g + 4half + e (g, e < 4): four runs of
 // four consecutive J entries, read as 8-byte words.  Row x is read at the
 // block's first two words; members further on (rare) read their own word.
-// Entries are counts below 2^11, so the f32 sum of 16 is exact.
+// Entries are counts of at most 2^11 (row width 2048), so the f32 sum of 16
+// is exact (checked by the static_assert in process_root).
 template <class RowT>
 __device__ __forceinline__ uint32_t edge_masked_sum(const RowT& R, const int16_t* J, int n, int whi, int Y, int x,
                                                     int half, const v16f32& acc) {
@@ -437,7 +438,7 @@ __device__ __forceinline__ v8i32 one_hot_fp4(int y, int q, int half) {
   return enc_a(in ? 1u << (y & 31) : 0u);
 }
 
-// Entries are counts below 2^11 and 0/1 masks: the f32 sum of 16 products
+// Entries are counts of at most 2^11 and 0/1 masks: the f32 sum of 16 products
 // is exact.
 __device__ __forceinline__ uint32_t dot16(const v16f32& c, const v16f32& m) {
   float s = 0.f;
@@ -691,6 +692,14 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
   // PHASE 0: build and count; 1: build only (rows left in rowp); 2: count
   // only (rowp already holds the rows of this root)
   static_assert(PHASE == 0 || (!BYTES && K >= 4), "split phases are for counting with rows");
+  // Exactness of the fp4 matrix-core counts (tri_mfma1, tri_mfma, tri4_mfma):
+  // every product of two set e2m1 elements is exactly 1, an accumulator entry
+  // counts at most 64 * WMAX positions, and the epilogues add 16 entries in
+  // f32.  Both stay integers below 2^24 (f32's exact range) only while
+  // 16 * 64 * WMAX < 2^24; the row-width tiers (WMAX <= 32) are far inside.
+  static_assert(16 * 64 * (long)WMAX < (1l << 24), "fp4 counts would leave f32's exact integer range");
+  static_assert(64 * WMAX <= 2048, "rows wider than 2048 positions: the 16-bit position hash and the "
+                                   "exactness notes of the fp4 epilogues assume at most 2048");
   using LY = Lay<G, WMAX, K, BYTES, LROWS, PHASE != 1, PHASE != 2>;
   constexpr int DM = LY::DM;
   constexpr bool NEED_ROWS = LY::ROWS;

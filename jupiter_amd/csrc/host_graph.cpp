@@ -6,6 +6,7 @@
 // .meta.txt/.vertex.bin/.edge.bin files).  Everything here is host code: it
 // prepares the inputs the device kernels count over.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -165,6 +166,44 @@ int write_csr(const jgpm_csr_s& c, const char* prefix) {
   return JGPM_OK;
 }
 
+int check_csr(const jgpm_csr_s& c, bool symmetric, int bad) {
+  const int64_t nv = c.nv;
+  if ((int64_t)c.rowptr.size() != nv + 1 || (int64_t)c.col.size() != c.nnz) return bad;
+  if (c.rowptr[0] != 0 || c.rowptr[nv] != c.nnz) return bad;
+  for (int64_t v = 0; v < nv; ++v)
+    if (c.rowptr[v + 1] < c.rowptr[v]) return bad;
+  std::atomic<bool> ok{true};
+  parallel_for(nv, 1024, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e && ok.load(std::memory_order_relaxed); ++v) {
+      const int32_t* l = c.col.data() + c.rowptr[v];
+      const int64_t d = c.rowptr[v + 1] - c.rowptr[v];
+      for (int64_t k = 0; k < d; ++k) {
+        const int32_t u = l[k];
+        if (u < 0 || u >= nv || u == v || (k > 0 && u <= l[k - 1])) { ok = false; break; }
+        if (symmetric && u > v) {
+          const int32_t* m = c.col.data() + c.rowptr[u];
+          if (!std::binary_search(m, c.col.data() + c.rowptr[u + 1], (int32_t)v)) { ok = false; break; }
+        }
+      }
+    }
+  });
+  if (!ok) return bad;
+  if (symmetric) {
+    // every u < v entry of a list has its mirror: with the u > v entries found
+    // above, symmetry holds iff the two halves have the same size
+    std::atomic<int64_t> lo{0}, hi{0};
+    parallel_for(nv, 4096, [&](int64_t b, int64_t e) {
+      int64_t a = 0, z = 0;
+      for (int64_t v = b; v < e; ++v)
+        for (int64_t k = c.rowptr[v]; k < c.rowptr[v + 1]; ++k) (c.col[k] < v ? a : z) += 1;
+      lo += a;
+      hi += z;
+    });
+    if (lo.load() != hi.load()) return bad;
+  }
+  return JGPM_OK;
+}
+
 int read_csr(const char* prefix, jgpm_csr_s& c) {
   std::string p(prefix);
   std::ifstream fm(p + ".meta.txt");
@@ -187,11 +226,10 @@ int read_csr(const char* prefix, jgpm_csr_s& c) {
   fv.read(reinterpret_cast<char*>(c.rowptr.data()), sizeof(int64_t) * (nv + 1));
   fe.read(reinterpret_cast<char*>(c.col.data()), sizeof(int32_t) * ne);
   if (!fv || !fe) return JGPM_E_IO;
-  if (c.rowptr[0] != 0 || c.rowptr[nv] != ne) return JGPM_E_IO;
-  for (int64_t v = 0; v < nv; ++v)
-    if (c.rowptr[v + 1] < c.rowptr[v]) return JGPM_E_IO;
-  for (int64_t k = 0; k < ne; ++k)
-    if (c.col[k] < 0 || c.col[k] >= nv) return JGPM_E_IO;
+  // a file may hold a DAG (Graph::dump_binary of an oriented graph), so only
+  // the list invariants are checked here; loaders that orient check symmetry
+  int st = check_csr(c, false, JGPM_E_IO);
+  if (st) return st;
   c.max_deg = compute_max_degree(c);
   return JGPM_OK;
 }

@@ -349,13 +349,10 @@ extern "C" int jgpm_graph_upload(jgpm_ctx_t c, const int64_t* rowptr, const int3
   } catch (const std::bad_alloc&) {
     return JGPM_E_NOMEM;
   }
-  for (int32_t v = 0; v < nv; ++v) {
-    if (h.rowptr[v + 1] < h.rowptr[v]) return JGPM_E_INVALID;
-    for (int64_t k = h.rowptr[v]; k < h.rowptr[v + 1]; ++k) {
-      if (h.col[k] < 0 || h.col[k] >= nv) return JGPM_E_INVALID;
-      if (k > h.rowptr[v] && h.col[k] <= h.col[k - 1]) return JGPM_E_INVALID;  // ascending, no duplicates
-    }
-  }
+  // ascending duplicate-free lists without loops; a symmetric upload must
+  // be symmetric, since the library derives the degree DAG from it
+  int st = check_csr(h, flags == JGPM_SYM, JGPM_E_INVALID);
+  if (st) return st;
   h.max_deg = compute_max_degree(h);
   if (flags == JGPM_DAG) return graph_from_host(c, nullptr, &h, out);
   jgpm_csr_s dag;
@@ -372,6 +369,8 @@ extern "C" int jgpm_graph_load(jgpm_ctx_t c, const char* prefix, int dag_only, j
   jgpm_csr_s h;
   int st = read_csr(prefix, h);
   if (st) return st;
+  // the file must hold the symmetric graph: the DAG is derived from it
+  if ((st = check_csr(h, true, JGPM_E_IO))) return st;
   jgpm_csr_s dag;
   try {
     orient_dag(h, dag);
@@ -545,6 +544,10 @@ extern "C" int jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64
   if (pattern < 0 || pattern >= JGPM_NUM_PATTERNS) return JGPM_E_PATTERN;
   int st = check_range(g, root_begin, root_end);
   if (st) return st;
+  // The induced motifs combine counts that credit an occurrence to different
+  // vertices (P2 to the larger edge end, P13 to the DAG source, the vertex
+  // sums to the centre), so only the whole graph gives an exact value.
+  if (pattern_slots(pattern) > 1 && (root_begin != 0 || root_end != g->dag.nv)) return JGPM_E_INVALID;
   jgpm_ctx_s* c = g->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   HIP_TRY(hipSetDevice(c->dev));

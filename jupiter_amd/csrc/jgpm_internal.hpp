@@ -66,6 +66,11 @@ JGPM_HD static inline void rmat_sample(int scale, const RmatCuts& t, uint64_t se
 }
 
 int build_sym_csr(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst, jgpm_csr_s& out);
+// Checks what every kernel assumes of a CSR: row pointers from 0 to nnz,
+// non-decreasing; ids in [0, nv); lists strictly ascending (sorted, no
+// duplicates) without self loops; with `symmetric`, v in N(u) iff u in N(v).
+// Returns JGPM_OK or `bad` (the caller's status for a malformed graph).
+int check_csr(const jgpm_csr_s& c, bool symmetric, int bad);
 void orient_dag(const jgpm_csr_s& sym, jgpm_csr_s& dag);
 int32_t compute_max_degree(const jgpm_csr_s& c);
 int write_csr(const jgpm_csr_s& c, const char* prefix);

@@ -590,6 +590,21 @@ int oracle_work_bytes(const int64_t* rowptr, const int32_t* col, int32_t nv, int
   return 0;
 }
 
+// Algorithmic bytes of the bitmap clique engine over the roots [rb, re) of a
+// DAG: every root reads its own list and each member's list once, at 16 + 4 d
+// bytes per list (two int64 row pointers plus the int32 ids).
+int oracle_design_bytes(const int64_t* rowptr, const int32_t* col, int32_t nv, int64_t rb, int64_t re,
+                        uint64_t* out) {
+  if (!rowptr || !out || rb < 0 || re > nv || rb > re) return -1;
+  acc_t total = 0;
+  for (int64_t v = rb; v < re; ++v) {
+    total += 16 + 4 * (acc_t)(rowptr[v + 1] - rowptr[v]);
+    for (int64_t e = rowptr[v]; e < rowptr[v + 1]; ++e) total += 16 + 4 * (acc_t)(rowptr[col[e] + 1] - rowptr[col[e]]);
+  }
+  *out = total;
+  return 0;
+}
+
 // Graph::orientation restated: returns the DAG nnz when out_col is NULL,
 // otherwise fills out_rowptr[nv+1] and out_col[nnz].
 int64_t oracle_orient(const int64_t* rowptr, const int32_t* col, int32_t nv, int64_t* out_rowptr,
