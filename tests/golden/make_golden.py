@@ -1,8 +1,12 @@
 """Writes tests/golden/counts.json: oracle counts for seeded graphs that are
 too slow to recompute inside a GPU test run.  Run from the repo root:
     python tests/golden/make_golden.py
-The oracle itself is pinned by tests/test_oracle.py (brute force, closed
-forms); these fixtures freeze its output for the larger graphs."""
+The oracle itself is pinned by tests/test_oracle.py and
+tests/test_oracle_pins.py (brute force, closed forms, per-root attribution);
+these fixtures freeze its output for the larger graphs.
+    python tests/golden/make_golden.py --rmat22
+writes tests/golden/rmat22.json (the benchmark graph: CSR hashes and root
+windows of its first quarter, about two minutes on 8 cores)."""
 import json
 import os
 import sys
@@ -12,13 +16,56 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
+import numpy as np  # noqa: E402
+
 import jupiter_amd as J  # noqa: E402
 import oracle_lib as O  # noqa: E402
 
 CLIQUE = [0, 13, 14, 15, 16]
 
 
+# Root windows of the R-MAT scale-22 graph (edge factor 16, seed 1) inside the
+# benchmark's quarter [0, 2^20), chosen so that together they hold roots of
+# every clique tier (DAG out-degree <= 64, 128, 256, 512, 1024) and the oracle
+# finishes each in under a minute.
+RMAT22_WINDOWS = [(917500, 918600), (600000, 604096), (300, 302), (1015000, 1016000), (1032000, 1033300),
+                  (4000, 4010)]
+
+
+def csr_sha256(rowptr, col):
+    import hashlib
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(rowptr, dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(col, dtype=np.int32).tobytes())
+    return h.hexdigest()
+
+
+def rmat22():
+    """tests/golden/rmat22.json: hashes of the host-built symmetric and DAG
+    CSR, per-tier root counts and oracle TC/P13/P14 counts of the windows."""
+    g = J.rmat(22, 16, seed=1)
+    d = g.orient()
+    dd = np.diff(d.rowptr)
+    ent = {"scale": 22, "edge_factor": 16, "abcd": [0.57, 0.19, 0.19, 0.05], "seed": 1, "nv": g.nv,
+           "sym_nnz": g.nnz, "dag_nnz": d.nnz, "dag_max_deg": int(dd.max()),
+           "sym_sha256": csr_sha256(g.rowptr, g.col), "dag_sha256": csr_sha256(d.rowptr, d.col), "windows": []}
+    for a, b in RMAT22_WINDOWS:
+        x = dd[a:b]
+        w = {"roots": [a, b], "tier_roots": {str(hi): int(((x > lo) & (x <= hi)).sum())
+                                              for lo, hi in ((0, 64), (64, 128), (128, 256), (256, 512), (512, 1024))}}
+        for p in (0, 13, 14):
+            t = time.time()
+            w[J.PATTERN_NAMES[p]] = O.count(d.rowptr, d.col, p, a, b)
+            print("rmat22", (a, b), J.PATTERN_NAMES[p], w[J.PATTERN_NAMES[p]], f"{time.time() - t:.1f}s", flush=True)
+        ent["windows"].append(w)
+    with open(os.path.join(HERE, "rmat22.json"), "w") as f:
+        json.dump(ent, f, indent=1, sort_keys=True)
+
+
 def main():
+    if "--rmat22" in sys.argv:
+        rmat22()
+        return
     out = {}
     for scale, seed, pats in [(14, 1, CLIQUE + [1, 2]), (12, 7, CLIQUE + list(range(1, 13)))]:
         g = J.rmat(scale, 16, seed=seed)
