@@ -6,7 +6,8 @@ tests/test_oracle_pins.py (brute force, closed forms, per-root attribution);
 these fixtures freeze its output for the larger graphs.
     python tests/golden/make_golden.py --rmat22
 writes tests/golden/rmat22.json (the benchmark graph: CSR hashes and root
-windows of its first quarter, about two minutes on 8 cores)."""
+windows of its first quarter and TC/P13 over the whole quarter, about ten
+minutes on 8 cores)."""
 import json
 import os
 import sys
@@ -58,6 +59,15 @@ def rmat22():
             w[J.PATTERN_NAMES[p]] = O.count(d.rowptr, d.col, p, a, b)
             print("rmat22", (a, b), J.PATTERN_NAMES[p], w[J.PATTERN_NAMES[p]], f"{time.time() - t:.1f}s", flush=True)
         ent["windows"].append(w)
+    # the whole first quarter for TC and P13 (about 7 minutes on 6 cores);
+    # P14 over the quarter would take hours here
+    q = g.nv // 4
+    ent["quarter0"] = {}
+    for p in (0, 13):
+        t = time.time()
+        ent["quarter0"][J.PATTERN_NAMES[p]] = O.count(d.rowptr, d.col, p, 0, q)
+        print("rmat22 quarter 0", J.PATTERN_NAMES[p], ent["quarter0"][J.PATTERN_NAMES[p]],
+              f"{time.time() - t:.1f}s", flush=True)
     with open(os.path.join(HERE, "rmat22.json"), "w") as f:
         json.dump(ent, f, indent=1, sort_keys=True)
 
