@@ -124,6 +124,18 @@ int  jgpm_graph_bytes(jgpm_graph_t g, int64_t* bytes);
  * whole graph only: any other range returns JGPM_E_INVALID. */
 int  jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
                 uint64_t* count, double* kernel_ms);
+/* jgpm_count for the clique patterns (JGPM_TC, P13..P16, JGPM_I_K4) without
+ * any device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The query's device scratch (accumulators,
+ * queue counters, rows of the 2048-wide tier) is a static 32 MiB array per
+ * GPU that the library holds until the process ends; the 2048-wide tier runs
+ * with as many blocks as it holds row slices.  Other pattern ids return
+ * JGPM_E_PATTERN, roots wider than 2048 JGPM_E_BUDGET (as jgpm_count).  The
+ * array is shared by every call on one GPU: calls through one context are
+ * serialised, calls through two contexts on the same GPU must not overlap. */
+int  jgpm_count_noalloc(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,
+                        uint64_t* count);
 /* Algorithmic bytes of the reference's canonical loop nest for the same
  * query: every neighbour-list fetch costs 16 + 4*d(v) bytes.  Computed on
  * the device.  Defined for JGPM_TC and P13..P16. */

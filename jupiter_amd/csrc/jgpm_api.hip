@@ -272,6 +272,12 @@ extern "C" int jgpm_init(int dev, jgpm_ctx_t* out) {
     return JGPM_E_NOMEM;
   }
   c->tune = default_tuning();
+  // the no-allocation entry point's scratch address on this device
+  int st = noalloc_setup(dev, nullptr);
+  if (st) {
+    (void)jgpm_shutdown(c);
+    return st;
+  }
   *out = c;
   return JGPM_OK;
 }

@@ -78,6 +78,9 @@ struct jgpm_graph_s {
 
 namespace jgpm {
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
+// Address of this device's static scratch array of jgpm_count_noalloc
+// (noalloc.hip), taken once per device with `dev` current.
+int noalloc_setup(int dev, uint64_t** out);
 // 4-cycle (P1) wedge kernel.  p1_counter_ints: the zeroed counter ints a
 // graph of nv vertices needs; ensure_p1_counters grows the context's (it
 // allocates, never while capturing); launch_p1 only launches, into *out, with
