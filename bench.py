@@ -1,30 +1,44 @@
-"""Benchmark: 4-clique (P13) and 5-clique (P14) counting over one quarter of
-the roots of a seeded R-MAT graph (scale 22, edge factor 16, a/b/c/d =
-0.57/0.19/0.19/0.05, degree-ordered DAG, int32 CSR, uint64 counts).
+"""Benchmark of the pattern-counting path on one MI355X.
 
-A step is one P13 query plus one P14 query over roots [0, 2^20), each a
-plain per-call launch that returns its exact count to the host.  `value` is
-queries/s over the timed steps (inputs resident in HBM before timing).
+The timed workload: 4-clique (P13) and 5-clique (P14) counting over one
+quarter of the roots of a seeded R-MAT graph (scale 22, edge factor 16,
+a/b/c/d = 0.57/0.19/0.19/0.05, degree-ordered DAG, int32 CSR, uint64
+counts).  A step is one P13 query plus one P14 query over roots [0, 2^20),
+each a plain per-call launch that returns its exact count to the host.
+`value` is queries/s over the timed steps (inputs resident in HBM before
+timing); the counts are checked against the oracle's frozen counts.
 
-Also reported on the same JSON line:
-  roofline      the dominant tier (largest device time over the timed
-                region, from HIP events on the library's stream; one launch,
-                or the build and count launches of the split 1024-wide tier):
-                work-model bytes of the tier (every neighbour-list fetch of
-                the reference's loop nest costs 16 + 4 d(v) bytes) / its
-                average time per query, against the 8 TB/s HBM peak;
-                `traffic` is its measured HBM bytes (rocprofv3 FETCH_SIZE +
-                WRITE_SIZE, profiles/);
+Also on the same JSON line:
+  roofline      the dominant clique tier (largest device time, HIP events on
+                the library's stream; for the split 1024-wide tier its build
+                and count launches): algorithmic bytes of the bitmap design
+                (every root reads its own list and each member's list once,
+                16 + 4 d+ bytes per list) / its average time per query, against
+                the 8 TB/s HBM peak; `traffic` is the measured HBM bytes of the
+                same launches (rocprofv3 FETCH_SIZE + WRITE_SIZE summarised in
+                profiles/), reported only while the kernel sources are the ones
+                that were measured;
+  other_configs the triangle count on R-MAT 14 (launch-bound latency) and
+                the vertex-induced 4-vertex motifs on the 3.8M-vertex
+                Chung-Lu graph, each checked against frozen oracle counts;
+  serving       Poisson request streams over 32 resident R-MAT graphs of
+                scales 18-24 (uniform graph, uniform pattern id P1..P16 within
+                the stated per-pattern caps), continuous batching vs the
+                plain one-query-per-launch path, equal counts required;
   cpu_baseline  the CPU oracle (clean-room OpenMP restatement of the
-                reference nests) timed on a seeded random sample of the same
-                roots, scaled to the full query;
-  latency       p50 / p99 per query kind over the timed steps.
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                reference nests) on a stratified sample of the same roots,
+                all host threads and one thread, scaled to the full query with
+                its standard error.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve]
+       python bench.py --workload batch|serve     (the batched / serving
+                                                    workloads on their own)
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -38,45 +52,131 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "queries/s served vs plain per-call kernel (bit-exact counts); p50/p99; GB/s"
 SCALE, EF, SEED = 22, 16, 1
 QUERIES = ("P13", "P14")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+TIER_NAMES = ("clique_wave64", "clique_lds128", "clique_lds256", "clique_lds512", "clique_lds1024", "clique_glb2048")
+TIER_MAX = (64, 128, 256, 512, 1024, 2048)
+CLIQUE_K = {"TC": 3, "P13": 4, "P14": 5, "P15": 6, "P16": 7}
+# kernel sources whose change makes a committed traffic measurement stale
+CLIQUE_SOURCES = ("jupiter_amd/csrc/clique_kernels.hip", "jupiter_amd/csrc/clique_dev.hpp")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dag, roots_end, frac_den=1024, seed=7):
-    """Oracle on a seeded uniform sample of 1/frac_den of the roots, both
-    query kinds; scaled by frac_den to the full query."""
+def source_digest(paths=CLIQUE_SOURCES):
+    h = hashlib.sha256()
+    for p in paths:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def design_bytes_by_tier(rowptr, col, rb, re, k):
+    """Algorithmic bytes of the bitmap clique engine per tier over the roots
+    [rb, re) of a DAG: a root of out-degree d in [k-1, 2048] reads its list and
+    each member's list once, 16 + 4 d+ bytes per list (two int64 row pointers
+    and the int32 ids); roots with fewer than k-1 members hold no k-clique and
+    are skipped by the kernels."""
+    rowptr = np.asarray(rowptr, dtype=np.int64)
+    d = np.diff(rowptr)
+    c = 16 + 4 * d
+    # sum over members of c, per root (empty rows give 0)
+    cs = np.concatenate([[0], np.cumsum(c[np.asarray(col, dtype=np.int64)])])
+    member = cs[rowptr[1:]] - cs[rowptr[:-1]]
+    roots = np.arange(rb, re)
+    dr = d[rb:re]
+    per_root = c[rb:re] + member[rb:re]
+    out = {}
+    lo = k - 1
+    for name, hi in zip(TIER_NAMES, TIER_MAX):
+        m = (dr >= lo) & (dr <= hi)
+        out[name] = {"roots": int(m.sum()), "bytes": int(per_root[m].sum())}
+        lo = hi + 1
+    del roots
+    return out
+
+
+def measured_traffic(launch_names, pattern):
+    """HBM bytes (read + written) of one pass of the given launches on the
+    default workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+    profiles/ (scaled by the plain-copy calibration).  None when a launch is
+    not measured or the clique sources changed since the measurement."""
+    path = os.path.join(ROOT, "profiles", "clique_p13_p14_rmat22_q0_traffic.json")
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))
+    if t.get("sources_sha256_16") != source_digest():
+        return None
+    tbl = t.get("kernels", {})
+    keys = [f"{n} ({pattern})" for n in launch_names]
+    if any(k not in tbl for k in keys):
+        return None
+    return int(sum(tbl[k]["read_bytes"] + tbl[k]["write_bytes"] for k in keys))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(dag, roots_end, seed=7, per_stratum=(256, 64, 32, 16, 8), batches=4):
+    """Oracle (P13 + P14) on a stratified random sample of the roots
+    [0, roots_end).  Strata are the clique tiers by DAG out-degree.  One
+    thread: each stratum's sample is timed in `batches` random batches; the
+    stratum's time is (its roots) x (mean time per sampled root), with the
+    standard error from the spread of the batch means.  All threads: the
+    whole sample in one dynamically scheduled call gives the parallel speedup
+    over the one-thread time of the same roots, which scales the one-thread
+    estimate."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     rng = np.random.default_rng(seed)
-    roots = np.sort(rng.choice(roots_end, size=roots_end // frac_den, replace=False)).astype(np.int32)
+    d = np.diff(dag.rowptr)[:roots_end]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    t_total = 0.0
-    for p in QUERIES:
+    bounds = [(0, 64), (65, 128), (129, 256), (257, 512), (513, 2048)]
+    strata = []
+    for (lo, hi), m in zip(bounds, per_stratum):
+        ids = np.nonzero((d >= lo) & (d <= hi))[0]
+        if len(ids) == 0:
+            continue
+        take = np.sort(rng.choice(ids, size=min(m, len(ids)), replace=False)).astype(np.int32)
+        strata.append((len(ids), take))
+
+    def timed(roots, nthreads):
         t0 = time.perf_counter()
-        O.count_roots(dag.rowptr, dag.col, J.PATTERNS[p], roots, mode=1, threads=threads)
-        t_total += time.perf_counter() - t0
-    est = t_total * frac_den  # seconds for one P13 + one P14 over all roots
-    return {"value": round(len(QUERIES) / est, 6), "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{len(roots)} roots (uniform seeded 1/{frac_den} of [0,{roots_end})), P13+P14, "
-                      f"{t_total:.1f}s measured, scaled x{frac_den}"}
+        for p in QUERIES:
+            O.count_roots(dag.rowptr, dag.col, J.PATTERNS[p], roots, mode=1, threads=nthreads)
+        return time.perf_counter() - t0
 
-
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "clique_p13_p14_rmat22_q0_traffic.json")
-
-
-def measured_traffic(kernels, scale):
-    """HBM bytes (read + written) of one pass of the launches `kernels` on
-    the default workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    summarised in profiles/ (scaled by the plain-copy calibration); None when
-    one of them is not measured."""
-    if scale != SCALE or not os.path.exists(TRAFFIC_FILE):
-        return None
-    tbl = json.load(open(TRAFFIC_FILE)).get("kernels", {})
-    if any(k not in tbl for k in kernels):
-        return None
-    return int(sum(tbl[k]["read_bytes"] + tbl[k]["write_bytes"] for k in kernels))
+    est1, var1, secs1 = 0.0, 0.0, 0.0
+    for n_s, take in strata:
+        means = []
+        for part in np.array_split(rng.permutation(take), batches):
+            dt = timed(np.sort(part), 1)
+            secs1 += dt
+            means.append(dt / max(1, len(part)))
+        means = np.array(means)
+        est1 += n_s * means.mean()
+        var1 += (n_s ** 2) * means.var(ddof=1) / len(means)
+    allroots = np.sort(np.concatenate([t for _, t in strata]))
+    wall = timed(allroots, threads)
+    speedup = secs1 / wall
+    se1 = float(np.sqrt(var1))
+    q = len(QUERIES)
+    return {"value": round(q * speedup / est1, 6), "unit": "queries/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"stratified by clique tier: {len(allroots)} of {roots_end} roots "
+                      f"({', '.join(str(len(t)) + '/' + str(n) for n, t in strata)}), P13+P14; one thread "
+                      f"{secs1:.1f}s, all {threads} threads {wall:.1f}s on the same roots",
+            "est_s_per_step": round(est1 / speedup, 1), "std_err_s": round(se1 / speedup, 1),
+            "parallel_speedup": round(speedup, 2),
+            "one_thread": {"value": round(q / est1, 7), "est_s_per_step": round(est1, 1), "std_err_s": round(se1, 1)}}
 
 
 def splitmix(seed, i):
@@ -85,45 +185,53 @@ def splitmix(seed, i):
     return (x * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
 
 
+def config4_specs(n_graphs=2048, max_n=262144):
+    """The batched workload's graphs: (kind, n, parameter, seed, pattern),
+    half Erdos-Renyi (mean degree U[8,64]), half Barabasi-Albert (m U{4..16}),
+    n log-uniform in [1k, max_n], pattern uniform from {TC, P13, P1, P2}."""
+    rng = np.random.default_rng(SEED)
+    out = []
+    for i in range(n_graphs):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(max_n))))
+        sd = splitmix(SEED, i)
+        if i % 2 == 0:
+            spec = ("ER", n, float(rng.uniform(8, 64)), sd)
+        else:
+            spec = ("BA", n, int(rng.integers(4, 17)), sd)
+        out.append(spec + (["TC", "P13", "P1", "P2"][int(rng.integers(0, 4))],))
+    return out
+
+
+def config4_graph(spec):
+    kind, n, par, sd, _ = spec
+    return J.erdos_renyi(n, par, seed=sd) if kind == "ER" else J.barabasi_albert(n, par, seed=sd)
+
+
 def workload_batch(args):
     """Many independent graphs, each paired with one pattern from
     {TC, P13, P1, P2}: the batch autotuned, captured once in a HIP graph and
     replayed, vs one plain launch per query.  The default is the full
     configuration (2,048 graphs, n up to 262,144)."""
-    rng = np.random.default_rng(SEED)
-    specs = []
+    specs = config4_specs(args.batch_graphs, args.batch_max_n)
     t0 = time.time()
     edges = 0
     ctx = J.Context(0)
     Gs = []
-    for i in range(args.batch_graphs):
-        n = int(np.exp(rng.uniform(np.log(1000), np.log(args.batch_max_n))))
-        sd = splitmix(SEED, i)
-        if i % 2 == 0:
-            md = float(rng.uniform(8, 64))
-            c = J.erdos_renyi(n, md, seed=sd)
-            kind = f"ER(n={n},deg={md:.1f})"
-        else:
-            m = int(rng.integers(4, 17))
-            c = J.barabasi_albert(n, m, seed=sd)
-            kind = f"BA(n={n},m={m})"
-        specs.append((kind, ["TC", "P13", "P1", "P2"][int(rng.integers(0, 4))]))
+    for i, spec in enumerate(specs):
+        c = config4_graph(spec)
         edges += c.nnz // 2
         Gs.append(ctx.upload(c))  # resident; the host copy is dropped
         if (i + 1) % 128 == 0:
             log(f"generated and uploaded {i + 1} graphs, {edges} edges ({time.time() - t0:.1f}s)")
     log(f"generated {len(Gs)} graphs, {edges} edges ({time.time() - t0:.1f}s)")
     with ctx:
-        queries = [(G, p, 0, None) for G, (_, p) in zip(Gs, specs)]
+        queries = [(G, s[4], 0, None) for G, s in zip(Gs, specs)]
         for _ in range(args.warmup):
             ctx.count_batch(queries)
-        # eager batched launch with the default knobs
         t0 = time.perf_counter()
         for _ in range(args.steps):
             eager_counts, _ = ctx.count_batch(queries)
         t_eager = (time.perf_counter() - t0) / args.steps
-        # autotune the knobs on this batch (not timed), then capture the batch
-        # in a HIP graph and time its replays
         t0 = time.perf_counter()
         tuned, tuned_ms = ctx.autotune_batch(queries, reps=2)
         t_tune = time.perf_counter() - t0
@@ -167,100 +275,176 @@ def workload_batch(args):
                        "speedup_vs_plain": round(t_plain / t_batch, 3), "counts_equal_plain": True}}
 
 
-SERVE_CAP = {"TC": 99, "P2": 99, "P13": 99, "P3": 99, "P5": 99, "P7": 99, "P9": 99, "P14": 99,
-             "P1": 14, "P11": 14, "P12": 14, "P4": 12, "P8": 12, "P15": 12, "P6": 11, "P10": 10, "P16": 10}
+# Largest R-MAT scale a served pattern draws its graph from (the request's
+# graph is uniform among the resident graphs within the cap).  Patterns that
+# no resident scale admits are not drawn; both are stated on the line.
+SERVE_CAP = {"P1": 24, "P2": 24, "P3": 22, "P13": 24, "P14": 24, "TC": 24,
+             "P5": 18, "P7": 18, "P9": 18, "P11": 18, "P12": 18, "P15": 18,
+             "P4": 0, "P6": 0, "P8": 0, "P10": 0, "P16": 0}
+SERVE_PATTERNS = [f"P{i}" for i in range(1, 17)]
 
 
-def workload_serve(args):
-    """Poisson request stream over resident R-MAT graphs, each request a
-    uniform pattern id (TC, P1..P16) on a uniform graph (within the
-    pattern's scale cap), served with continuous batching vs sequential plain
-    launches (FIFO, one query per launch).  Counts must be identical."""
-    import threading
+def serve_graph_scales(min_scale=18, max_scale=24, n=32):
     rng = np.random.default_rng(SEED)
-    scales = [int(rng.integers(args.serve_min_scale, args.serve_max_scale + 1)) for _ in range(32)]
-    pats = ["TC"] + [f"P{i}" for i in range(1, 17)]
-    out = {"rates": {}}
-    with J.Context(0) as ctx:
-        t0 = time.time()
-        Gs = [ctx.rmat_graph(sc, EF, 0.57, 0.19, 0.19, seed=splitmix(SEED, i)) for i, sc in enumerate(scales)]
-        log(f"built 32 R-MAT graphs on the device, scales {sorted(scales)} ({time.time() - t0:.1f}s)")
-        for rate in args.rates:
-            reqs = []
-            for _ in range(args.requests):
-                p = pats[int(rng.integers(0, len(pats)))]
-                ok = [i for i, sc in enumerate(scales) if sc <= SERVE_CAP[p]]
-                reqs.append((int(rng.choice(ok)), p))
-            arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
-            # served
-            res = [None] * len(reqs)
-            with J.Server(ctx) as srv:
-                tickets = [None] * len(reqs)
-                start = time.perf_counter()
+    return [int(rng.integers(min_scale, max_scale + 1)) for _ in range(n)]
 
-                def submitter():
-                    for k, (gi, p) in enumerate(reqs):
-                        dt = arrivals[k] - (time.perf_counter() - start)
-                        if dt > 0:
-                            time.sleep(dt)
-                        tickets[k] = srv.submit(Gs[gi], p)
-                th = threading.Thread(target=submitter)
-                th.start()
-                th.join()
-                for k in range(len(reqs)):
-                    res[k] = srv.wait(tickets[k])
-                span = time.perf_counter() - start
-            lat = np.array([r[1] for r in res]) / 1e3
-            # sequential plain launches, FIFO from the same arrival times
-            plain = []
-            plat = []
-            start = time.perf_counter()
+
+def serve_requests(scales, n_req, rate, seed):
+    """Poisson arrivals at `rate`; each request a uniform pattern id of
+    P1..P16 among those a resident scale admits, on a uniform graph within the
+    pattern's cap."""
+    rng = np.random.default_rng(seed)
+    pats = [p for p in SERVE_PATTERNS if any(sc <= SERVE_CAP[p] for sc in scales)]
+    reqs = []
+    for _ in range(n_req):
+        p = pats[int(rng.integers(0, len(pats)))]
+        ok = [i for i, sc in enumerate(scales) if sc <= SERVE_CAP[p]]
+        reqs.append((int(rng.choice(ok)), p))
+    return reqs, np.cumsum(rng.exponential(1.0 / rate, n_req)), pats
+
+
+def run_served(ctx, Gs, reqs, arrivals):
+    res = [None] * len(reqs)
+    with J.Server(ctx) as srv:
+        tickets = [None] * len(reqs)
+        start = time.perf_counter()
+
+        def submitter():
             for k, (gi, p) in enumerate(reqs):
                 dt = arrivals[k] - (time.perf_counter() - start)
                 if dt > 0:
                     time.sleep(dt)
-                c, _ = Gs[gi].count(p)
-                plain.append(c)
-                plat.append((time.perf_counter() - start - arrivals[k]) * 1e3)
-            pspan = time.perf_counter() - start
-            if [r[0] for r in res] != plain:
-                raise AssertionError(f"served counts differ from plain counts at rate {rate}")
-            out["rates"][str(rate)] = {
-                "served_qps": round(len(reqs) / span, 2), "served_p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "served_p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "plain_qps": round(len(reqs) / pspan, 2),
-                "plain_p50_ms": round(float(np.percentile(plat, 50)), 3),
-                "plain_p99_ms": round(float(np.percentile(plat, 99)), 3)}
-            log(f"rate {rate}: {out['rates'][str(rate)]}")
+                tickets[k] = srv.submit(Gs[gi], p)
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        for k in range(len(reqs)):
+            res[k] = srv.wait(tickets[k])
+        span = time.perf_counter() - start
+    return [r[0] for r in res], np.array([r[1] for r in res]) / 1e3, span
+
+
+def run_plain(Gs, reqs, arrivals):
+    counts, lat = [], []
+    start = time.perf_counter()
+    for k, (gi, p) in enumerate(reqs):
+        dt = arrivals[k] - (time.perf_counter() - start)
+        if dt > 0:
+            time.sleep(dt)
+        c, _ = Gs[gi].count(p)
+        counts.append(c)
+        lat.append((time.perf_counter() - start - arrivals[k]) * 1e3)
+    return counts, np.array(lat), time.perf_counter() - start
+
+
+def pct(a, q):
+    return round(float(np.percentile(a, q)), 3)
+
+
+def serving(ctx, rates, n_req, min_scale=18, max_scale=24):
+    """The serving configuration: 32 resident R-MAT graphs (ef 16), Poisson
+    arrivals at each rate; served (continuous batching) and plain (FIFO, one
+    query per launch) runs of the same request stream; counts must agree."""
+    scales = serve_graph_scales(min_scale, max_scale)
+    t0 = time.time()
+    Gs = [ctx.rmat_graph(sc, EF, 0.57, 0.19, 0.19, seed=splitmix(SEED, i)) for i, sc in enumerate(scales)]
+    log(f"serving: built 32 R-MAT graphs on the device, scales {sorted(scales)} ({time.time() - t0:.1f}s)")
+    # warm every (graph, pattern) pair's lazily built arrays once, untimed
+    rates_out = {}
+    pats = None
+    try:
+        warm = set()
+        for rate in rates:
+            reqs, arr, pats = serve_requests(scales, n_req, rate, seed=int(rate))
+            for gi, p in reqs:
+                if (gi, p) not in warm:
+                    Gs[gi].count(p, (0, 1))
+                    warm.add((gi, p))
+            log(f"serving rate {rate}: {len(reqs)} requests")
+            sc, slat, sspan = run_served(ctx, Gs, reqs, arr)
+            pc, plat, pspan = run_plain(Gs, reqs, arr)
+            if sc != pc:
+                bad = [i for i, (x, y) in enumerate(zip(sc, pc)) if x != y]
+                raise AssertionError(f"served counts differ from plain counts at rate {rate}: {bad[:10]}")
+            rates_out[str(rate)] = {
+                "served_qps": round(len(reqs) / sspan, 2), "served_p50_ms": pct(slat, 50),
+                "served_p99_ms": pct(slat, 99), "plain_qps": round(len(reqs) / pspan, 2),
+                "plain_p50_ms": pct(plat, 50), "plain_p99_ms": pct(plat, 99),
+                "served_over_plain": round(pspan / sspan, 3)}
+            log(f"serving rate {rate}: {rates_out[str(rate)]}")
+    finally:
         for G in Gs:
             G.free()
-    best = max(out["rates"].values(), key=lambda r: r["served_qps"])
-    return {"metric": METRIC, "value": best["served_qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(args.rates),
-            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "int32", "data": "synthetic",
-            "config": {"workload": f"serving: 32 resident R-MAT graphs scales {args.serve_min_scale}-"
-                                   f"{args.serve_max_scale} (ef 16), Poisson arrivals, uniform pattern id "
-                                   "(TC,P1..P16) with per-pattern scale caps, continuous batching vs "
-                                   "sequential plain launches", "scale_caps": SERVE_CAP,
-                       "requests_per_rate": args.requests, **out}}
+    best = max(rates_out.values(), key=lambda r: r["served_qps"])
+    return {"workload": "32 resident R-MAT graphs (ef 16, 0.57/0.19/0.19/0.05), scales "
+                        f"{min_scale}-{max_scale} uniform; Poisson arrivals; request = uniform pattern id of P1..P16 "
+                        "on a uniform graph within the pattern's scale cap; continuous batching vs FIFO plain launches",
+            "graph_scales": sorted(scales), "patterns_drawn": pats,
+            "scale_caps": {p: SERVE_CAP[p] for p in SERVE_PATTERNS},
+            "not_drawn": [p for p in SERVE_PATTERNS if p not in (pats or [])],
+            "requests_per_rate": n_req, "rates": rates_out,
+            "best_served_qps": best["served_qps"], "best_served_over_plain": best["served_over_plain"]}
+
+
+def other_configs(ctx):
+    """Triangle count on R-MAT 14 (latency, launch-bound) and the induced
+    4-vertex motifs on the 3.8M-vertex Chung-Lu graph, each against the
+    oracle's frozen counts."""
+    out = {}
+    gold = json.load(open(os.path.join(GOLDEN, "counts.json")))["rmat14_s1"]["counts"]
+    G = ctx.rmat_graph(14, 16, 0.57, 0.19, 0.19, seed=1)
+    for _ in range(5):
+        G.count("TC")
+    lat, kms = [], []
+    for _ in range(100):
+        t0 = time.perf_counter()
+        c, ms = G.count("TC")
+        lat.append((time.perf_counter() - t0) * 1e3)
+        kms.append(ms)
+        assert c == gold["TC"], (c, gold["TC"])
+    G.free()
+    out["tc_rmat14"] = {"workload": "TC on R-MAT scale 14 (ef 16, seed 1), whole graph, one range",
+                        "count": gold["TC"], "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                        "kernel_ms": {"p50": pct(kms, 50), "p99": pct(kms, 99)}}
+    cl = json.load(open(os.path.join(GOLDEN, "chunglu.json")))
+    t0 = time.time()
+    g = J.chung_lu(cl["n"], cl["target_edges"], cl["gamma"], cl["max_weight"], seed=cl["seed"])
+    G = ctx.upload(g)
+    gen_s = time.time() - t0
+    del g
+    res = {}
+    for p in ("I_C4", "I_DIAMOND", "I_TAILTRI", "I_STAR3", "P1", "P2", "P13"):
+        G.count(p)  # builds the per-graph arrays once
+        t0 = time.perf_counter()
+        c, ms = G.count(p)
+        wall = (time.perf_counter() - t0) * 1e3
+        assert c == cl["counts"][p], (p, c, cl["counts"][p])
+        res[p] = {"count": c, "kernel_ms": round(ms, 3), "wall_ms": round(wall, 3)}
+    G.free()
+    out["induced_chunglu"] = {"workload": f"vertex-induced 4-vertex motifs, Chung-Lu n={cl['nv']} "
+                                          f"({cl['sym_nnz'] // 2} edges, gamma 2.3, max weight 800, seed 1)",
+                              "host_build_s": round(gen_s, 2), "queries": res}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-other", action="store_true")
     ap.add_argument("--scale", type=int, default=SCALE)
     ap.add_argument("--profile", action="store_true",
                     help="only the counting kernels of one step (for rocprofv3 counter runs)")
     ap.add_argument("--workload", choices=["cliques", "batch", "serve"], default="cliques")
     ap.add_argument("--batch-graphs", type=int, default=2048)
     ap.add_argument("--batch-max-n", type=int, default=262144)
-    ap.add_argument("--serve-min-scale", type=int, default=10)
-    ap.add_argument("--serve-max-scale", type=int, default=16)
-    ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 2000])
-    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--serve-min-scale", type=int, default=18)
+    ap.add_argument("--serve-max-scale", type=int, default=24)
+    ap.add_argument("--rates", type=float, nargs="+", default=[200, 2000])
+    ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one plain 1 GiB device-to-device copy kernel (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -276,13 +460,15 @@ def main():
         print("bench.py measures one GPU: there is no multi-GPU path (a count shards by root range, "
               "each GPU adding its partial count). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-
     if args.workload == "batch":
         print(json.dumps(workload_batch(args)), flush=True)
         return
     if args.workload == "serve":
-        print(json.dumps(workload_serve(args)), flush=True)
+        with J.Context(0) as ctx:
+            print(json.dumps(serving(ctx, args.rates, args.requests, args.serve_min_scale, args.serve_max_scale)),
+                  flush=True)
         return
+    gold = json.load(open(os.path.join(GOLDEN, "rmat22.json")))
     with J.Context(0) as ctx:
         # the graph is sampled, symmetrised, deduplicated and oriented on the
         # GPU (identical to the host construction), untimed
@@ -320,44 +506,65 @@ def main():
         elapsed = time.perf_counter() - t_start
         n_queries = args.steps * len(QUERIES)
         value = n_queries / elapsed
-        # dominant tier and its work-model bytes.  A tier is one launch, or
-        # two for the split 1024-wide tier (its "_build" launch writes the
-        # rows that the count launch reads): its time is the sum of its
-        # launches' average times
+        # counts against the oracle's frozen counts (whole quarter for P13;
+        # P14 over the quarter is checked by its root windows in the tests)
+        checked = {}
+        if args.scale == SCALE:
+            for p, want in gold.get("quarter0", {}).items():
+                if p in counts:
+                    if counts[p] != want:
+                        raise AssertionError(f"{p} count {counts[p]} differs from the oracle's {want}")
+                    checked[p] = "oracle, whole quarter"
+        # dominant tier: one launch, or the build and count launches of the
+        # split 1024-wide tier; its time is the sum of their average times
         tiers = {}
         for (p, i, n), v in kern.items():
             tiers.setdefault((p, n[:-len("_build")] if n.endswith("_build") else n), []).append((n, float(np.mean(v))))
         (dp, dname), dk = max(tiers.items(), key=lambda kv: sum(ms for _, ms in kv[1]))
         avg_ms = sum(ms for _, ms in dk)
-        G.count(dp, (0, q))
-        names = [n for n, _ in ctx.last_profile() if not n.endswith("_build")]
-        # the work model runs the fused per-tier kernels: match by tier name
-        by_name = dict(zip(names, G.work_bytes_split(dp, (0, q))))
-        dbytes = by_name[dname]
-        achieved = dbytes / (avg_ms * 1e-3) / 1e9
-        total_bytes = {p: G.work_bytes(p, (0, q)) for p in QUERIES}
-        dag = None if args.no_cpu else G.download(dag=True)
+        dag = G.download(dag=True)
         G.free()
-    kname = f"{dname} ({dp})"
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic([f"{n} ({dp})" for n, _ in dk], args.scale),
-                "kernel": kname, "launches": {n: round(ms, 3) for n, ms in dk}, "kernel_avg_ms": round(avg_ms, 3),
-                "algorithmic_bytes": dbytes}
-    line = {
-        "metric": METRIC, "value": round(value, 4), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"P13+P14 count, roots [0,2^{args.scale - 2}) of R-MAT scale {args.scale} "
-                               f"ef {EF} (0.57/0.19/0.19/0.05, seed {SEED}), degree DAG",
-                   "nv": info["nv"], "sym_nnz": info["sym_nnz"], "dag_nnz": info["dag_nnz"],
-                   "dag_max_deg": info["dag_max_deg"], "counts": counts,
-                   "latency_ms": {p: {"p50": round(float(np.percentile(v, 50)), 3),
-                                      "p99": round(float(np.percentile(v, 99)), 3)} for p, v in lat.items()},
-                   "kernels_ms": {f"{p}:{n}": round(float(np.mean(v)), 3) for (p, i, n), v in kern.items()},
-                   "work_bytes": total_bytes, "device_ingest_s": round(ingest_s, 3)},
-        "roofline": roofline,
-    }
+        t0 = time.time()
+        per_tier = {p: design_bytes_by_tier(dag.rowptr, dag.col, 0, q, CLIQUE_K[p]) for p in QUERIES}
+        log(f"design bytes per tier ({time.time() - t0:.1f}s): {per_tier}")
+        dbytes = per_tier[dp][dname]["bytes"]
+        achieved = dbytes / (avg_ms * 1e-3) / 1e9
+        if achieved > HBM_PEAK_GBS:
+            raise AssertionError(f"design bytes exceed the HBM peak: {achieved} GB/s")
+        traffic = measured_traffic([n for n, _ in dk], dp)
+        query_ms = {p: float(np.mean([sum(v[k] for (pp, _, _), v in kern.items() if pp == p)
+                                      for k in range(args.steps)])) for p in QUERIES}
+        query_bytes = {p: sum(t["bytes"] for t in per_tier[p].values()) for p in QUERIES}
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": f"{dname} ({dp})", "launches": {n: round(ms, 3) for n, ms in dk},
+                    "kernel_avg_ms": round(avg_ms, 3), "algorithmic_bytes": dbytes,
+                    "roots": per_tier[dp][dname]["roots"],
+                    "per_query": {p: {"design_bytes": query_bytes[p], "device_ms": round(query_ms[p], 3),
+                                      "GBps": round(query_bytes[p] / query_ms[p] / 1e6, 1),
+                                      "frac": round(query_bytes[p] / query_ms[p] / 1e6 / HBM_PEAK_GBS, 4)}
+                                  for p in QUERIES},
+                    "traffic_source": "profiles/clique_p13_p14_rmat22_q0_traffic.json" if traffic else
+                                      "not measured for these kernel sources"}
+        line = {
+            "metric": METRIC, "value": round(value, 4), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": f"P13+P14 count, roots [0,2^{args.scale - 2}) of R-MAT scale {args.scale} "
+                                   f"ef {EF} (0.57/0.19/0.19/0.05, seed {SEED}), degree DAG",
+                       "nv": info["nv"], "sym_nnz": info["sym_nnz"], "dag_nnz": info["dag_nnz"],
+                       "dag_max_deg": info["dag_max_deg"], "counts": counts, "counts_checked": checked,
+                       "latency_ms": {p: {"p50": pct(v, 50), "p99": pct(v, 99)} for p, v in lat.items()},
+                       "kernels_ms": {f"{p}:{n}": round(float(np.mean(v)), 3) for (p, i, n), v in kern.items()},
+                       "device_ingest_s": round(ingest_s, 3)},
+            "roofline": roofline,
+        }
+        if not args.no_other:
+            log("other configurations ...")
+            line["other_configs"] = other_configs(ctx)
+        if not args.no_serve:
+            log("serving ...")
+            line["serving"] = serving(ctx, args.rates, args.requests, args.serve_min_scale, args.serve_max_scale)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(dag, q)

@@ -590,6 +590,50 @@ int oracle_work_bytes(const int64_t* rowptr, const int32_t* col, int32_t nv, int
   return 0;
 }
 
+// Non-induced 3-stars (mode 0: sum_v C(d_v, 3)) and tailed triangles
+// (mode 1: sum_v t_v (d_v - 2), t_v the triangles at v) over the centres
+// v in [rb, re) of a symmetric CSR: the extension ids of the induced-motif
+// workload (textbook closed forms); each triangle
+// u < w < x is listed once by a merge of N(u) and N(w) above w.
+int oracle_vertex_sums(const int64_t* rowptr, const int32_t* col, int32_t nv, int mode, int64_t rb, int64_t re,
+                       uint64_t* out) {
+  if (!rowptr || !out || rb < 0 || re > nv || rb > re || (mode != 0 && mode != 1)) return -1;
+  G g{rowptr, col, nv};
+  std::vector<acc_t> t(mode == 1 ? (size_t)nv : 0, 0);
+  if (mode == 1) {
+    for (vid u = 0; u < nv; ++u) {
+      const vid* Nu = g.N(u);
+      const vid du = g.d(u);
+      for (vid i = 0; i < du; ++i) {
+        const vid w = Nu[i];
+        if (w <= u) continue;
+        // x in N(u) ∩ N(w), x > w
+        const vid* Nw = g.N(w);
+        vid a = 0, b = 0;
+        const vid dw = g.d(w);
+        while (a < du && b < dw) {
+          const vid x = Nu[a], y = Nw[b];
+          if (x <= y) ++a;
+          if (y <= x) ++b;
+          if (x == y && x > w) {
+            t[u]++;
+            t[w]++;
+            t[x]++;
+          }
+        }
+      }
+    }
+  }
+  acc_t total = 0;
+  for (int64_t v = rb; v < re; ++v) {
+    const acc_t d = (acc_t)g.d((vid)v);
+    if (mode == 0) total += d >= 3 ? d * (d - 1) * (d - 2) / 6 : 0;
+    else total += d >= 2 ? t[v] * (d - 2) : 0;
+  }
+  *out = total;
+  return 0;
+}
+
 // Algorithmic bytes of the bitmap clique engine over the roots [rb, re) of a
 // DAG: every root reads its own list and each member's list once, at 16 + 4 d
 // bytes per list (two int64 row pointers plus the int32 ids).

@@ -7,7 +7,9 @@ these fixtures freeze its output for the larger graphs.
     python tests/golden/make_golden.py --rmat22
 writes tests/golden/rmat22.json (the benchmark graph: CSR hashes and root
 windows of its first quarter and TC/P13 over the whole quarter, about ten
-minutes on 8 cores)."""
+minutes on 8 cores).
+    python tests/golden/make_golden.py --chunglu
+writes tests/golden/chunglu.json (the induced-motif workload graph)."""
 import json
 import os
 import sys
@@ -72,9 +74,41 @@ def rmat22():
         json.dump(ent, f, indent=1, sort_keys=True)
 
 
+def chung_lu_config():
+    """tests/golden/chunglu.json: the induced-motif workload graph (Chung-Lu,
+    3.8M vertices, 16.5M target edges, gamma 2.3, max weight 800, seed 1):
+    its size, a CSR hash and the oracle's non-induced counts the induced
+    motifs are made of (P1, P2, P13, 3-star, tailed triangle)."""
+    g = J.chung_lu(3_800_000, 16_500_000, 2.3, 800.0, seed=1)
+    d = g.orient()
+    ent = {"n": 3_800_000, "target_edges": 16_500_000, "gamma": 2.3, "max_weight": 800.0, "seed": 1,
+           "nv": g.nv, "sym_nnz": g.nnz, "max_deg": int(g.degrees().max()), "sym_sha256": csr_sha256(g.rowptr, g.col),
+           "counts": {}}
+    c = ent["counts"]
+    for name, fn in (("P1", lambda: O.count(g.rowptr, g.col, 1)), ("P2", lambda: O.count(g.rowptr, g.col, 2)),
+                     ("P13", lambda: O.count(d.rowptr, d.col, 13)), ("TC", lambda: O.count(d.rowptr, d.col, 0)),
+                     ("STAR3", lambda: O.vertex_sums(g.rowptr, g.col, 0)),
+                     ("TAILTRI", lambda: O.vertex_sums(g.rowptr, g.col, 1))):
+        t = time.time()
+        c[name] = fn()
+        print("chunglu", name, c[name], f"{time.time() - t:.1f}s", flush=True)
+    # the vertex-induced motifs by the exact inverse conversion
+    S, T, C, D, K = c["STAR3"], c["TAILTRI"], c["P1"], c["P2"], c["P13"]
+    c["I_K4"] = K
+    c["I_DIAMOND"] = D - 6 * K
+    c["I_C4"] = C - D + 3 * K
+    c["I_TAILTRI"] = T - 4 * D + 12 * K
+    c["I_STAR3"] = S - T + 2 * D - 4 * K
+    with open(os.path.join(HERE, "chunglu.json"), "w") as f:
+        json.dump(ent, f, indent=1, sort_keys=True)
+
+
 def main():
     if "--rmat22" in sys.argv:
         rmat22()
+        return
+    if "--chunglu" in sys.argv:
+        chung_lu_config()
         return
     out = {}
     for scale, seed, pats in [(14, 1, CLIQUE + [1, 2]), (12, 7, CLIQUE + list(range(1, 13)))]:

@@ -29,6 +29,8 @@ def lib():
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
         L.oracle_design_bytes.argtypes = [P, P, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
                                           ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_vertex_sums.argtypes = [P, P, ctypes.c_int32, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_uint64)]
         L.oracle_orient.argtypes = [P, P, ctypes.c_int32, P, P]
         L.oracle_orient.restype = ctypes.c_int64
         L.oracle_isect.argtypes = [P, ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32, P]
@@ -74,6 +76,19 @@ def work_bytes(rowptr, col, pid, rb=0, re=None, threads=0):
         re = nv
     out = ctypes.c_uint64(0)
     rc = lib().oracle_work_bytes(_p(rowptr), _p(col), nv, pid, rb, re, threads, ctypes.byref(out))
+    assert rc == 0, rc
+    return out.value
+
+
+def vertex_sums(rowptr, col, mode, rb=0, re=None):
+    """mode 0: 3-stars sum C(d,3); mode 1: tailed triangles sum t_v (d_v - 2)."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    nv = len(rowptr) - 1
+    if re is None:
+        re = nv
+    out = ctypes.c_uint64(0)
+    rc = lib().oracle_vertex_sums(_p(rowptr), _p(col), nv, mode, rb, re, ctypes.byref(out))
     assert rc == 0, rc
     return out.value
 

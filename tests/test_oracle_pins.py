@@ -340,3 +340,30 @@ def test_design_bytes_is_tc_work():
         want = sum(int(c[v]) + int(c[dcol[drp[v]:drp[v + 1]]].sum()) for v in range(rb, re))
         assert O.work_bytes(drp, dcol, 0, rb, re) == want
         assert O.design_bytes(drp, dcol, rb, re) == want
+
+
+# ---------------------------------------------------------------------------
+# vertex sums (3-star, tailed triangle)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_vertex_sums_against_brute_force(seed):
+    import brute
+    rng = random.Random(300 + seed)
+    rp, col, adj = graphs.random_graph(rng.randint(5, 10), rng.uniform(0.3, 0.8), rng)
+    assert O.vertex_sums(rp, col, 0) == brute.count_pattern(adj, 17)
+    assert O.vertex_sums(rp, col, 1) == brute.count_pattern(adj, 18)
+
+
+def test_vertex_sums_sparse_closed_form():
+    import scipy.sparse as sp
+    g = J.chung_lu(20000, 70000, 2.3, 300.0, seed=5)
+    A = sp.csr_matrix((np.ones(g.nnz), g.col, g.rowptr), shape=(g.nv, g.nv))
+    tv = np.asarray((A @ A).multiply(A).sum(axis=1)).ravel().astype(np.int64) // 2
+    deg = g.degrees().astype(np.int64)
+    assert O.vertex_sums(g.rowptr, g.col, 0) == int((deg * (deg - 1) * (deg - 2) // 6).sum())
+    assert O.vertex_sums(g.rowptr, g.col, 1) == int((tv * (deg - 2)).sum())
+    q = g.nv // 4
+    for mode in (0, 1):
+        assert sum(O.vertex_sums(g.rowptr, g.col, mode, i * q, (i + 1) * q) for i in range(4)) == \
+            O.vertex_sums(g.rowptr, g.col, mode)
