@@ -278,7 +278,11 @@ def workload_batch(args):
 # Largest R-MAT scale a served pattern draws its graph from (the request's
 # graph is uniform among the resident graphs within the cap).  Patterns that
 # no resident scale admits are not drawn; both are stated on the line.
-SERVE_CAP = {"P1": 24, "P2": 24, "P3": 22, "P13": 24, "P14": 24, "TC": 24,
+# The caps keep one plain query within about a second (measured whole-graph
+# times on one MI355X: P1 0.78 s at scale 20 and 9.2 s at 22; P2 1.27 s at 22
+# and 9.4 s at 24; P3 0.39 s at 20; P5/P7/P9/P11/P12/P15 1.6-3.6 s at 18; P4
+# 2.4 s at 16; P6, P8, P10 and P16 take seconds already at scales 12-16).
+SERVE_CAP = {"P1": 20, "P2": 22, "P3": 20, "P13": 24, "P14": 24, "TC": 24,
              "P5": 18, "P7": 18, "P9": 18, "P11": 18, "P12": 18, "P15": 18,
              "P4": 0, "P6": 0, "P8": 0, "P10": 0, "P16": 0}
 SERVE_PATTERNS = [f"P{i}" for i in range(1, 17)]

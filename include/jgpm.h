@@ -259,7 +259,9 @@ int  jgpm_serve_start(jgpm_ctx_t ctx, const jgpm_serve_config* cfg, jgpm_serve_t
 int  jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, int64_t root_begin,
                        int64_t root_end, uint64_t* ticket);
 /* Non-blocking: JGPM_E_NOTREADY until the ticket's count is available.
- * latency_us (may be NULL): submit -> result-visible time. */
+ * latency_us (may be NULL): submit -> result-visible time.  A result is
+ * handed out once (poll or wait); the server then forgets the ticket, and a
+ * second retrieval returns JGPM_E_INVALID. */
 int  jgpm_serve_poll(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us);
 int  jgpm_serve_wait(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us);
 int  jgpm_serve_stop(jgpm_serve_t s);

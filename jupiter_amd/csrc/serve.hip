@@ -14,6 +14,12 @@
 //    runs them all in one persistent batched launch; requests that arrived
 //    meanwhile are admitted before the next launch, so their work overlaps
 //    with the tails of earlier ones on the device;
+//  * heavy requests: a request whose work (edges of its range times the
+//    pattern weight) exceeds one slice would fill the device by itself, so
+//    it is not sliced: it runs whole with the single-query kernels, between
+//    the batched launches of the light requests (its count is then exactly
+//    jgpm_count's, from the same kernels, including the degree-order
+//    relabelling a whole-graph 4-cycle count uses);
 //  * completion: a request finishes when its last slice is counted; its
 //    latency is submit -> result visible on the host.
 // Counts are sums over disjoint root slices, so a result is bit-identical to
@@ -27,6 +33,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "jgpm_device.hpp"
@@ -42,6 +49,7 @@ struct Req {
   int64_t rb, re, cur;
   uint64_t count = 0;
   int64_t footprint = 0;
+  bool heavy = false;  // runs whole, with the single-query kernels (not sliced into batches)
   clk::time_point t_submit;
 };
 
@@ -73,7 +81,8 @@ struct jgpm_serve_s {
   std::condition_variable cv_work, cv_done;
   std::deque<Req> pending;
   std::vector<Req> active;
-  std::unordered_map<uint64_t, Done> done;
+  std::unordered_map<uint64_t, Done> done;  // finished, not yet retrieved
+  std::unordered_set<uint64_t> live;         // submitted, not yet retrieved
   uint64_t next_ticket = 1;
   int64_t inflight_bytes = 0;
   bool stop = false;
@@ -118,12 +127,17 @@ static void serve_loop(jgpm_serve_s* s) {
         s->pending.pop_front();
       }
     }
-    // next slice of every active request (the active list is only touched here)
+    // next slice of every light active request (the active list is only touched here)
     qs.clear();
     owner.clear();
+    int heavy = -1;
     for (size_t i = 0; i < s->active.size(); ++i) {
       Req& r = s->active[i];
       if (r.cur >= r.re) continue;
+      if (r.heavy) {
+        if (heavy < 0) heavy = (int)i;
+        continue;
+      }
       jgpm_graph_s* g = r.g;
       const std::vector<int64_t>& rp = is_dag_pattern(r.pattern) ? g->dag_rp_h : g->sym_rp_h;
       const int64_t budget = std::max<int64_t>(1, (int64_t)(s->slice_edges / pattern_weight(r.pattern)));
@@ -150,9 +164,29 @@ static void serve_loop(jgpm_serve_s* s) {
         st = JGPM_E_HIP;
       if (hipStreamSynchronize(c->st) != hipSuccess && !st) st = JGPM_E_HIP;
     }
+    // then the oldest heavy request, whole, with the single-query kernels
+    uint64_t heavy_count = 0;
+    if (!st && heavy >= 0) {
+      Req& r = s->active[heavy];
+      std::lock_guard<std::mutex> lk(c->mu);
+      st = (hipSetDevice(c->dev) == hipSuccess) ? JGPM_OK : JGPM_E_HIP;
+      if (!st) st = jgpm::ctx_reset_counters(c, c->st);
+      if (!st) {
+        c->nprof = 0;
+        st = jgpm::enqueue_pattern(r.g, r.pattern, r.rb, r.re, 0, c->st);
+      }
+      uint64_t hc[CTX_SLOTS];
+      if (!st && hipMemcpyAsync(hc, c->d_out, sizeof(hc), hipMemcpyDeviceToHost, c->st) != hipSuccess) st = JGPM_E_HIP;
+      if (hipStreamSynchronize(c->st) != hipSuccess && !st) st = JGPM_E_HIP;
+      if (!st) heavy_count = jgpm::finish_pattern(r.pattern, hc);
+    }
     const auto now = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
     for (size_t k = 0; k < qs.size(); ++k) s->active[owner[k]].count += h[k];
+    if (!st && heavy >= 0) {
+      s->active[heavy].count = heavy_count;
+      s->active[heavy].cur = s->active[heavy].re;
+    }
     // finished requests (or all, after an error) leave the device
     std::vector<Req> keep;
     for (Req& r : s->active) {
@@ -207,6 +241,10 @@ extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, in
   r.re = re;
   r.cur = rb;
   r.footprint = footprint(g, pattern, s->ctx, grid);
+  {
+    const std::vector<int64_t>& rp = is_dag_pattern(pattern) ? g->dag_rp_h : g->sym_rp_h;
+    r.heavy = (double)(rp[re] - rp[rb]) * pattern_weight(pattern) > (double)s->slice_edges;
+  }
   r.t_submit = clk::now();
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return JGPM_E_STATE;
@@ -214,6 +252,7 @@ extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, in
   if (r.footprint > s->cfg.hbm_budget_bytes) return JGPM_E_BUDGET;
   r.ticket = s->next_ticket++;
   *ticket = r.ticket;
+  s->live.insert(r.ticket);
   if (rb == re) {  // empty range: done at once
     s->done[r.ticket] = Done{0, 0.0, JGPM_OK};
     s->cv_done.notify_all();
@@ -224,27 +263,34 @@ extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, in
   return JGPM_OK;
 }
 
+// A result is handed out once: retrieving it (poll or wait) erases its
+// entry, so a long-running server keeps only unretrieved results.  A ticket
+// that is unknown or already retrieved is JGPM_E_INVALID.
+static int take_result(jgpm_serve_s* s, uint64_t ticket, uint64_t* count, double* latency_us) {
+  auto it = s->done.find(ticket);
+  const Done d = it->second;
+  s->done.erase(it);
+  s->live.erase(ticket);
+  if (d.status) return d.status;
+  *count = d.count;
+  if (latency_us) *latency_us = d.latency_us;
+  return JGPM_OK;
+}
+
 extern "C" int jgpm_serve_poll(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us) {
   if (!s || !count) return JGPM_E_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->done.find(ticket);
-  if (it == s->done.end()) return ticket == 0 || ticket >= s->next_ticket ? JGPM_E_INVALID : JGPM_E_NOTREADY;
-  if (it->second.status) return it->second.status;
-  *count = it->second.count;
-  if (latency_us) *latency_us = it->second.latency_us;
-  return JGPM_OK;
+  if (!s->live.count(ticket)) return JGPM_E_INVALID;
+  if (!s->done.count(ticket)) return JGPM_E_NOTREADY;
+  return take_result(s, ticket, count, latency_us);
 }
 
 extern "C" int jgpm_serve_wait(jgpm_serve_t s, uint64_t ticket, uint64_t* count, double* latency_us) {
   if (!s || !count) return JGPM_E_INVALID;
   std::unique_lock<std::mutex> lk(s->mu);
-  if (ticket == 0 || ticket >= s->next_ticket) return JGPM_E_INVALID;
+  if (!s->live.count(ticket)) return JGPM_E_INVALID;
   s->cv_done.wait(lk, [&] { return s->done.count(ticket) > 0; });
-  const Done& d = s->done[ticket];
-  if (d.status) return d.status;
-  *count = d.count;
-  if (latency_us) *latency_us = d.latency_us;
-  return JGPM_OK;
+  return take_result(s, ticket, count, latency_us);
 }
 
 extern "C" int jgpm_serve_stop(jgpm_serve_t s) {

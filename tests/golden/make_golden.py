@@ -9,7 +9,9 @@ writes tests/golden/rmat22.json (the benchmark graph: CSR hashes and root
 windows of its first quarter and TC/P13 over the whole quarter, about ten
 minutes on 8 cores).
     python tests/golden/make_golden.py --chunglu
-writes tests/golden/chunglu.json (the induced-motif workload graph)."""
+writes tests/golden/chunglu.json (the induced-motif workload graph).
+    python tests/golden/make_golden.py --batch2048
+writes tests/golden/batch2048.json (a sample of the batched workload)."""
 import json
 import os
 import sys
@@ -103,7 +105,37 @@ def chung_lu_config():
         json.dump(ent, f, indent=1, sort_keys=True)
 
 
+def config4_sample():
+    """tests/golden/batch2048.json: oracle counts for a sample of the batched
+    workload's 2,048 graphs (the 16 smallest and 4 of median size), each with
+    its own pattern, so a GPU run of the whole batch can be checked against
+    the oracle without recomputing it there."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    import bench
+    specs = bench.config4_specs()
+    order = sorted(range(len(specs)), key=lambda i: specs[i][1])
+    pick = sorted(order[:16] + order[len(order) // 2: len(order) // 2 + 4])
+    out = {"n_graphs": len(specs), "sample": []}
+    for i in pick:
+        g = bench.config4_graph(specs[i])
+        p = specs[i][4]
+        t = time.time()
+        if p in ("TC", "P13"):
+            d = g.orient()
+            c = O.count(d.rowptr, d.col, J.PATTERNS[p])
+        else:
+            c = O.count(g.rowptr, g.col, J.PATTERNS[p])
+        out["sample"].append({"index": i, "kind": specs[i][0], "n": specs[i][1], "pattern": p, "sym_nnz": g.nnz,
+                              "count": c})
+        print("batch2048", i, specs[i][:3], p, c, f"{time.time() - t:.1f}s", flush=True)
+    with open(os.path.join(HERE, "batch2048.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 def main():
+    if "--batch2048" in sys.argv:
+        config4_sample()
+        return
     if "--rmat22" in sys.argv:
         rmat22()
         return

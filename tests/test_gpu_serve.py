@@ -46,6 +46,23 @@ def test_mixed_concurrent_requests_equal_alone(ctx):
         G.free()
 
 
+def test_heavy_and_light_requests_mixed(ctx):
+    """Requests whose work exceeds one slice run whole with the single-query
+    kernels between the batched launches of the light ones; both kinds in one
+    stream of submissions give their counts alone."""
+    big = ctx.upload(J.rmat(16, 16, seed=5))      # P3, P5, P13, P14, P1 whole-graph: heavy
+    small = ctx.upload(J.rmat(10, 16, seed=6))    # everything light
+    reqs = [(big, "P3"), (small, "P2"), (big, "P13"), (small, "P5"), (big, "P1"), (small, "TC"),
+            (big, "P14"), (small, "P9"), (big, "TC"), (big, "P1", (100, 5000)), (small, "P13")]
+    alone = [r[0].count(r[1], r[2] if len(r) > 2 else None)[0] for r in reqs]
+    with J.Server(ctx) as srv:
+        tickets = [srv.submit(r[0], r[1], r[2] if len(r) > 2 else None) for r in reqs]
+        got = [srv.wait(t)[0] for t in tickets]
+    assert got == alone
+    big.free()
+    small.free()
+
+
 def test_budget_and_pattern_rejection(ctx):
     G = ctx.upload(J.rmat(10, 16, seed=1))
     with J.Server(ctx, hbm_budget_bytes=1 << 16) as srv:
@@ -57,4 +74,11 @@ def test_budget_and_pattern_rejection(ctx):
         assert e.value.status == -2
         t = srv.submit(G, "TC", (5, 5))
         assert srv.wait(t) == (0, 0.0)
+        # a result is handed out once, then the ticket is forgotten
+        with pytest.raises(J.JgpmError) as e:
+            srv.wait(t)
+        assert e.value.status == -1
+        with pytest.raises(J.JgpmError) as e:
+            srv.poll(t)
+        assert e.value.status == -1
     G.free()
