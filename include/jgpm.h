@@ -130,8 +130,8 @@ int  jgpm_count(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_en
  * counterpart in the reference.  The query's device scratch (accumulators,
  * queue counters, rows of the 2048-wide tier) is a static 32 MiB array per
  * GPU that the library holds until the process ends; the 2048-wide tier runs
- * with as many blocks as it holds row slices.  Other pattern ids return
- * JGPM_E_PATTERN, roots wider than 2048 JGPM_E_BUDGET (as jgpm_count).  The
+ * with as many blocks as it holds row slices, and roots wider than 2048 keep
+ * their level lists there too.  Other pattern ids return JGPM_E_PATTERN.  The
  * array is shared by every call on one GPU: calls through one context are
  * serialised, calls through two contexts on the same GPU must not overlap. */
 int  jgpm_count_noalloc(jgpm_graph_t g, int pattern, int64_t root_begin, int64_t root_end,

@@ -25,7 +25,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 LIB_SOURCES = ["host_graph.cpp", "clique_kernels.hip", "sym_kernels.hip", "jgpm_api.hip", "dispatch.hip",
-               "batch.hip", "serve.hip", "ingest.hip", "plan.cpp", "plan_kernels.hip", "noalloc.hip",
+               "batch.hip", "serve.hip", "ingest.hip", "plan.cpp", "plan_kernels.hip", "noalloc.hip", "clique_huge.hip",
                # the reference-shaped C++ host API (Graph, VertexSet, Pattern, SubgraphSolver)
                "Graph.cc", "VertexSet.cc", "Pattern.cc", "solver.cc"]
 CLI_SOURCES = ["main.cc"]

@@ -399,6 +399,7 @@ extern "C" int jgpm_graph_free(jgpm_graph_t g) {
   if (g->d_tri) (void)hipFree(g->d_tri);
   if (g->d_nsum) (void)hipFree(g->d_nsum);
   if (g->d_ord) (void)hipFree(g->d_ord);
+  if (g->huge_d) (void)hipFree(g->huge_d);
   free_csr(g->rsym);
   delete g;
   return JGPM_OK;
@@ -448,8 +449,45 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
                    size_t gscr_words) {
   jgpm_ctx_s* c = g->ctx;
   if (!g->huge_h.empty()) {
+    // roots wider than 2048 (the DAG upload keeps them in huge_h): the
+    // level-nest kernel, its lists in the row scratch
     auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
-    if (lo != g->huge_h.end() && *lo < re) return JGPM_E_BUDGET;
+    auto hi = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)std::min<int64_t>(re, INT32_MAX));
+    const int64_t n = hi - lo;
+    if (n > 0) {
+      if (bytes) return JGPM_E_BUDGET;  // the work model covers the row tiers only
+      const int64_t stride = std::max(1, g->dag.max_deg);
+      const int64_t per_block = (int64_t)jk::huge_lists_per_block(k) * stride;  // ints
+      const int64_t cap_ints = gscr ? (int64_t)(gscr_words * 2) : ((int64_t)1 << 26);  // 256 MiB of own scratch
+      int grid = (int)std::min<int64_t>({n, (int64_t)2 * c->num_cu, cap_ints / per_block});
+      if (grid < 1) return JGPM_E_BUDGET;
+      int32_t* scr = nullptr;
+      if (gscr) {
+        scr = reinterpret_cast<int32_t*>(gscr);
+      } else {
+        int s2 = ensure_scratch(c, (size_t)((grid * per_block + 1) / 2));
+        if (s2) return s2;
+        scr = reinterpret_cast<int32_t*>(c->d_scr);
+      }
+      if (!g->huge_d) {
+        if (c->capturing) return JGPM_E_STATE;
+        if (hipMalloc(&g->huge_d, g->huge_h.size() * sizeof(int32_t)) != hipSuccess) return JGPM_E_NOMEM;
+        HIP_TRY(hipMemcpy(g->huge_d, g->huge_h.data(), g->huge_h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        g->bytes += (int64_t)(g->huge_h.size() * sizeof(int32_t));
+      }
+      jk::HugeArgs h{};
+      h.rp = g->dag.rp;
+      h.ci = g->dag.ci;
+      h.ids = g->huge_d + (lo - g->huge_h.begin());
+      h.ntasks = n;
+      h.scr = scr;
+      h.stride = stride;
+      h.out = out_ptr ? out_ptr : c->d_out + out_slot;
+      prof_begin(c, st, "clique_huge");
+      hipError_t e = jk::launch_clique_huge(k, h, grid, st);
+      prof_end(c, st);
+      if (e != hipSuccess) return JGPM_E_HIP;
+    }
   }
   const int maxgrid = std::max(1, c->tune.blocks_per_cu) * c->num_cu;
   for (int tier = jk::NUM_TIERS - 1; tier >= first_tier; --tier) {

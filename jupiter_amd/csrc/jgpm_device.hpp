@@ -64,7 +64,8 @@ struct jgpm_graph_s {
   std::vector<int32_t> tier_h[jk::NUM_TIERS];  // root ids per block tier (ascending)
   int32_t* tier_d[jk::NUM_TIERS] = {nullptr};
   int32_t* d_ord = nullptr;                    // topological position of every vertex in the DAG
-  std::vector<int32_t> huge_h;                 // roots beyond the largest tier
+  std::vector<int32_t> huge_h;                 // roots beyond the largest tier (ascending)
+  int32_t* huge_d = nullptr;                   // their device copy (first count that needs it)
   std::vector<int64_t> sym_rp_h;               // host copy of the symmetric row pointers
   std::vector<int64_t> dag_rp_h;               // host copy of the DAG row pointers
   int32_t sym_mmin = 1;                        // max over edges of min(d(u), d(v)): longest S = N(u) ∩ N(v)

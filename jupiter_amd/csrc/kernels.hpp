@@ -107,4 +107,18 @@ hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb,
 int tier_max_deg(int tier);
 size_t tier_scratch_words(int tier);
 
+// Roots wider than the widest row tier (DAG out-degree > 2048): the level
+// nest per root, lists in global scratch (clique_huge.hip).
+struct HugeArgs {
+  const int64_t* rp;     // DAG row pointers
+  const int32_t* ci;     // DAG column indices
+  const int32_t* ids;    // the roots
+  int64_t ntasks;
+  int32_t* scr;          // huge_lists_per_block(k) * grid lists of stride ints
+  int64_t stride;        // >= the DAG's maximum out-degree
+  uint64_t* out;
+};
+int huge_lists_per_block(int k);
+hipError_t launch_clique_huge(int k, const HugeArgs& a, int grid, hipStream_t st);
+
 }  // namespace jk

@@ -5,9 +5,10 @@
 // and walks the plan depth first.  Level l's candidates are the intersection
 // of the neighbour lists of its constraining earlier levels, starting from
 // the shortest list, trimmed by binary search to the id range that the
-// symmetry-breaking restrictions leave, then filtered: earlier vertices are
-// removed (injectivity) and, for vertex-induced plans, so are neighbours of
-// the non-adjacent earlier levels.  The wave-wide set operations are the ones
+// symmetry-breaking restrictions leave, then filtered: for vertex-induced
+// plans the neighbours of the non-adjacent earlier levels are removed by a
+// wave-wide set difference (w_idiff), and earlier vertices are removed
+// (injectivity).  The wave-wide set operations are the ones
 // of the symmetric kernels (sym_dev.hpp).  The last level is counted without
 // materialising when it has at most two constraining lists: |A ∩ B| minus the
 // earlier vertices it contains.  Levels are a compile-time recursion, so the
@@ -39,10 +40,14 @@ struct PCtx {
   __device__ int d(int32_t v) const { return (int)(rp[v + 1] - rp[v]); }
 };
 
-// In place: keep x unless it equals an earlier level's vertex or (nadj) is
-// adjacent to the vertex of a level in nadj.
+// In place: drop the neighbours of every non-adjacent earlier level (nadj),
+// one set difference X − N(v_p) per such level, then every element equal to
+// an earlier level's vertex (injectivity).
 template <int L>
-__device__ int w_filter(const PCtx& c, int32_t* X, int n, const int32_t (&v)[PLAN_MAXK], uint32_t nadj) {
+__device__ int w_filter(const PCtx& c, Wave& w, int32_t* X, int n, const int32_t (&v)[PLAN_MAXK], uint32_t nadj) {
+#pragma unroll
+  for (int p = 0; p < L; ++p)
+    if ((nadj >> p) & 1u) n = w_idiff(w, X, n, c.N(v[p]), c.d(v[p]), INT32_MAX, X);
   int m = 0;
   for (int base = 0; base < n; base += 64) {
     const int i = base + lane();
@@ -52,10 +57,7 @@ __device__ int w_filter(const PCtx& c, int32_t* X, int n, const int32_t (&v)[PLA
       x = X[i];
       keep = true;
 #pragma unroll
-      for (int p = 0; p < L; ++p) {
-        keep = keep && x != v[p];
-        if ((nadj >> p) & 1u) keep = keep && !in_sorted(c.N(v[p]), c.d(v[p]), x);
-      }
+      for (int p = 0; p < L; ++p) keep = keep && x != v[p];
     }
     const uint64_t b = __ballot(keep);
     if (keep)
@@ -150,7 +152,7 @@ __device__ uint64_t plan_level(const PCtx& c, Wave& w, const Plan& P, int32_t (&
       }
     }
   }
-  nc = w_filter<L>(c, out, nc, v, nadj);
+  nc = w_filter<L>(c, w, out, nc, v, nadj);
   if (last) return (uint64_t)nc;
   uint64_t t = 0;
   if constexpr (L + 1 < PLAN_MAXK) {

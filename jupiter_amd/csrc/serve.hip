@@ -226,9 +226,11 @@ extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, in
   if (re < 0) re = nv;
   if (rb < 0 || rb > re || re > nv) return JGPM_E_INVALID;
   if (!is_dag_pattern(pattern) && !g->has_sym) return JGPM_E_GRAPHKIND;
+  // roots wider than 2048 only run in the single-query kernels
+  bool wide = false;
   if (is_dag_pattern(pattern) && !g->huge_h.empty()) {
     auto lo = std::lower_bound(g->huge_h.begin(), g->huge_h.end(), (int32_t)rb);
-    if (lo != g->huge_h.end() && *lo < re) return JGPM_E_BUDGET;
+    wide = lo != g->huge_h.end() && *lo < re;
   }
   // grid of the batched kernel (its occupancy is known after the first launch)
   const int occ = s->ctx->batch_occ > 0 ? s->ctx->batch_occ : 3;
@@ -243,7 +245,7 @@ extern "C" int jgpm_serve_submit(jgpm_serve_t s, jgpm_graph_t g, int pattern, in
   r.footprint = footprint(g, pattern, s->ctx, grid);
   {
     const std::vector<int64_t>& rp = is_dag_pattern(pattern) ? g->dag_rp_h : g->sym_rp_h;
-    r.heavy = (double)(rp[re] - rp[rb]) * pattern_weight(pattern) > (double)s->slice_edges;
+    r.heavy = wide || (double)(rp[re] - rp[rb]) * pattern_weight(pattern) > (double)s->slice_edges;
   }
   r.t_submit = clk::now();
   std::lock_guard<std::mutex> lk(s->mu);

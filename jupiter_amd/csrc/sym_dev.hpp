@@ -118,6 +118,44 @@ __device__ int w_imat(Wave& w, const int32_t* A, int na, const int32_t* B, int n
   return n;
 }
 
+// out = A − B restricted to values < ub: the elements of A (ascending) that B
+// (ascending) does not hold, in order; out may be A itself (compaction in
+// place).  A − ∅ = A (the reference's GPU difference_num / difference_set
+// return 0 for an empty B, include/set_difference.cuh:44,99; no P-id reaches
+// that case, and the vertex-induced plans need the set difference itself).
+// Lanes take 64 elements of A at a time and binary-search B, staged in LDS
+// when it is short; ballot + mbcnt keep the order.  Returns |out|.
+__device__ int w_idiff(Wave& w, const int32_t* A, int na, const int32_t* B, int nb, int32_t ub, int32_t* out) {
+  if (ub != INT32_MAX) {
+    na = lb(A, na, ub);
+    nb = lb(B, nb, ub);
+  }
+  const int32_t* S = B;
+  if (nb > 0 && nb <= STAGE && na > 4) {
+    for (int k = lane(); k < nb; k += 64) w.stage[k] = B[k];
+    __syncwarp();
+    S = w.stage;
+  }
+  int m = 0;
+  for (int base = 0; base < na; base += 64) {
+    const int i = base + lane();
+    bool keep = false;
+    int32_t x = 0;
+    if (i < na) {
+      x = A[i];
+      const int p = nb > 0 ? lb(S, nb, x) : 0;
+      keep = !(p < nb && S[p] == x);
+    }
+    const uint64_t b = __ballot(keep);
+    if (keep)
+      out[m + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] = x;
+    m += __popcll(b);
+  }
+  __syncwarp();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  return m;
+}
+
 __device__ __forceinline__ bool in_sorted(const int32_t* a, int n, int32_t x) {
   int p = lb(a, n, x);
   return p < n && a[p] == x;

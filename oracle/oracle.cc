@@ -683,5 +683,19 @@ int32_t oracle_isect(const int32_t* a, int32_t na, const int32_t* b, int32_t nb,
 uint64_t oracle_isect_num(const int32_t* a, int32_t na, const int32_t* b, int32_t nb, int32_t upper) {
   return isect_num(a, na, b, nb, upper);
 }
+// a − b restricted to values < upper (merge), the semantics of the CPU
+// VertexSet::difference_buf without the owner-vertex drop
+// (common/VertexSet.cc:21-89).  The GPU difference_* of the reference return
+// 0 for an empty b (include/set_difference.cuh:44,99,166,242); this restates
+// the set difference itself, a − ∅ = a, which the library implements.
+int32_t oracle_diff(const int32_t* a, int32_t na, const int32_t* b, int32_t nb, int32_t upper, int32_t* out) {
+  int32_t n = 0, i = 0, j = 0;
+  while (i < na && a[i] < upper) {
+    if (j >= nb || a[i] < b[j]) out[n++] = a[i++];
+    else if (a[i] == b[j]) { ++i; ++j; }
+    else ++j;
+  }
+  return n;
+}
 
 }  // extern "C"

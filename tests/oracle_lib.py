@@ -37,6 +37,8 @@ def lib():
         L.oracle_isect.restype = ctypes.c_int32
         L.oracle_isect_num.argtypes = [P, ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32]
         L.oracle_isect_num.restype = ctypes.c_uint64
+        L.oracle_diff.argtypes = [P, ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32, P]
+        L.oracle_diff.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -128,3 +130,11 @@ def isect_num(a, b, upper=2**31 - 1):
     a = np.ascontiguousarray(a, dtype=np.int32)
     b = np.ascontiguousarray(b, dtype=np.int32)
     return lib().oracle_isect_num(_p(a), len(a), _p(b), len(b), upper)
+
+
+def diff(a, b, upper=2**31 - 1):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    out = np.zeros(max(1, len(a)), dtype=np.int32)
+    n = lib().oracle_diff(_p(a), len(a), _p(b), len(b), upper, _p(out))
+    return out[:n]

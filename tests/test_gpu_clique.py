@@ -137,3 +137,34 @@ def test_dag_only_upload_uses_topological_order(ctx):
         _say(f"dag-only {p}")
         assert G.count(p)[0] == O.count(d.rowptr, d.col, J.PATTERNS[p]), p
     G.free()
+
+
+def test_roots_wider_than_2048(ctx):
+    """Roots with DAG out-degree above 2048 (beyond the widest row tier) run
+    the level nest in global scratch: a 2,600-wide sparse hub, and K_2100
+    (52 such roots) for triangles; counts against the oracle / closed form,
+    whole graph, the hub's root alone and through the serving loop."""
+    from math import comb
+    g = hub_graph(2600, 0.006, seed=26)
+    d = g.orient()
+    assert d.degrees()[0] == 2600
+    G = ctx.upload(g)
+    try:
+        for pat in CLIQUES:
+            _say(f"hub core=2600 {pat}")
+            want = O.count(d.rowptr, d.col, J.PATTERNS[pat])
+            assert G.count(pat)[0] == want, pat
+            assert G.count(pat, (0, 1))[0] == O.count(d.rowptr, d.col, J.PATTERNS[pat], 0, 1), pat
+        with J.Server(ctx) as srv:
+            t = srv.submit(G, "P14")
+            assert srv.wait(t)[0] == O.count(d.rowptr, d.col, J.PATTERNS["P14"])
+    finally:
+        G.free()
+    n = 2100
+    src, dst = np.triu_indices(n, 1)
+    G = ctx.upload(J.CSR.from_edges(n, src, dst))
+    try:
+        _say("K2100 TC")
+        assert G.count("TC")[0] == comb(n, 3)
+    finally:
+        G.free()

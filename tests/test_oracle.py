@@ -117,3 +117,5 @@ def test_set_primitives(seed):
         ref = sorted(x for x in set(a.tolist()) & set(b.tolist()) if x < upper)
         assert O.isect(a, b, upper).tolist() == ref
         assert O.isect_num(a, b, upper) == len(ref)
+        # set difference: A − B, with A − ∅ = A
+        assert O.diff(a, b, upper).tolist() == sorted(x for x in set(a.tolist()) - set(b.tolist()) if x < upper)
