@@ -193,6 +193,8 @@ static int batch_grid(jgpm_ctx_s* c) {
 // auxiliary arrays on st (kernels and allocations), so it is never captured.
 static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>& hq, std::vector<int64_t>& cs,
                       int32_t& max_sym, size_t& p1_ints, hipStream_t st) {
+  // cs holds two prefixes of n + 1 entries: the persistent kernel's chunks,
+  // then the roots of the 4-cycle queries (one batched wedge launch)
   const int rchunk = std::max(1, c->tune.batch_root_chunk);
   const int echunk = std::max(1, c->tune.batch_edge_chunk);
   hq.assign(n, jk::BQ{});
@@ -214,12 +216,15 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
     } else if (kind == 101) {
       // 4-cycles run as wedge launches after the persistent kernel (no chunks here)
       p1_ints = std::max(p1_ints, p1_counter_ints(c, (int32_t)g->sym.nv));
-      if (rb == 0 && re == g->sym.nv) {
+      const bool whole = rb == 0 && re == g->sym.nv;
+      if (whole) {
         int rc = ensure_rank_csr(g, st);
         if (rc) return rc;
       }
-      b.rp = g->sym.rp;
-      b.ci = g->sym.ci;
+      // a whole-graph count runs on the degree-ordered relabelling (same
+      // total, hubs ranked last), as the single-query launch does
+      b.rp = whole && g->rsym.ci ? g->rsym.rp : g->sym.rp;
+      b.ci = whole && g->rsym.ci ? g->rsym.ci : g->sym.ci;
       b.base = rb;
       b.ntasks = 0;
       b.chunk = 1;
@@ -238,8 +243,17 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
       max_sym = std::max(max_sym, g->sym_mmin);
     }
   }
-  cs.assign(n + 1, 0);
+  cs.assign(2 * (size_t)(n + 1), 0);
   for (int i = 0; i < n; ++i) cs[i + 1] = cs[i] + (hq[i].ntasks + hq[i].chunk - 1) / hq[i].chunk;
+  int64_t* p1cs = cs.data() + n + 1;
+  for (int i = 0; i < n; ++i) {
+    int64_t r = 0;
+    if (hq[i].kind == 101) {
+      const jgpm_graph_s* g = q[i].graph;
+      r = (q[i].root_end < 0 ? g->sym.nv : q[i].root_end) - q[i].root_begin;
+    }
+    p1cs[i + 1] = p1cs[i] + r;
+  }
   return JGPM_OK;
 }
 
@@ -267,14 +281,38 @@ static int launch_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, const std::ve
   prof_end(c, st);
   if (hipGetLastError() != hipSuccess) return JGPM_E_HIP;
   const int first_tier = hub <= 64 ? jk::TIER_LDS2 : jk::TIER_LDS4;
-  for (int i = 0; i < n; ++i) {
-    if (hq[i].kind == 101) {  // 4-cycles: wedge launch into the query's slot
-      jgpm_graph_s* g = q[i].graph;
-      const int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->sym.nv : q[i].root_end;
-      int rc = launch_p1(g, rb, re, b.bcnt + i, b.bq32 + (size_t)jk::NUM_TIERS * i, b.p1cnt, b.p1cnt_ints, st);
-      if (rc) return rc;
-      continue;
+  // every 4-cycle query in one wedge launch over their flattened roots
+  {
+    int first = -1;
+    int64_t total = 0;
+    int32_t max_nv = 1;
+    for (int i = 0; i < n; ++i)
+      if (hq[i].kind == 101) {
+        if (first < 0) first = i;
+        const jgpm_graph_s* g = q[i].graph;
+        total += (q[i].root_end < 0 ? g->sym.nv : q[i].root_end) - q[i].root_begin;
+        max_nv = std::max(max_nv, (int32_t)g->sym.nv);
+      }
+    if (total > 0) {
+      const int64_t waves = std::min<int64_t>((int64_t)c->num_cu * 32, (int64_t)(b.p1cnt_ints / (size_t)max_nv));
+      const int grid = (int)std::min<int64_t>(waves / jk::P1_WPB, (total + jk::P1_WPB - 1) / jk::P1_WPB);
+      if (!b.p1cnt || grid < 1) return JGPM_E_STATE;  // counters not sized for these graphs
+      jk::P1BatchArgs pa{};
+      pa.q = b.bq;
+      pa.p1cs = b.bcs + (n + 1);
+      pa.nq = n;
+      pa.total = total;
+      pa.qctr = b.bq32 + (size_t)jk::NUM_TIERS * first;  // zeroed above, unused by the wedge query otherwise
+      pa.cnt = b.p1cnt;
+      pa.cnt_stride = max_nv;
+      pa.counts = b.bcnt;
+      prof_begin(c, st, "batch_p1_wedge");
+      hipError_t e = jk::launch_p1_batch(pa, grid, st);
+      prof_end(c, st);
+      if (e != hipSuccess) return JGPM_E_HIP;
     }
+  }
+  for (int i = 0; i < n; ++i) {
     if (hq[i].kind >= 100) continue;
     jgpm_graph_s* g = q[i].graph;
     int64_t rb = q[i].root_begin, re = q[i].root_end < 0 ? g->dag.nv : q[i].root_end;
@@ -299,7 +337,7 @@ static int ensure_batch_buffers(jgpm_ctx_s* c, int n) {
   c->bcap = 0;
   int cap = std::max(64, n);
   if (hipMalloc(&c->d_bq, sizeof(jk::BQ) * cap) != hipSuccess ||
-      hipMalloc(&c->d_bcs, sizeof(int64_t) * (cap + 1)) != hipSuccess ||
+      hipMalloc(&c->d_bcs, sizeof(int64_t) * 2 * (cap + 1)) != hipSuccess ||
       hipMalloc(&c->d_bcnt, sizeof(uint64_t) * cap) != hipSuccess ||
       hipMalloc(&c->d_bq32, sizeof(unsigned) * jk::NUM_TIERS * cap) != hipSuccess)
     return JGPM_E_NOMEM;
@@ -330,7 +368,7 @@ int enqueue_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk::BQ>
     c->sscr_ints = need;
   }
   if (hipMemcpyAsync(c->d_bq, hq.data(), sizeof(jk::BQ) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(c->d_bcs, cs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess)
+      hipMemcpyAsync(c->d_bcs, cs.data(), sizeof(int64_t) * 2 * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess)
     return JGPM_E_HIP;
   BatchBufs b;
   b.bq = static_cast<jk::BQ*>(c->d_bq);
@@ -503,7 +541,7 @@ extern "C" int jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* q, int n, jgp
   const size_t sscr = (size_t)B->grid * jk::BWPB * jk::symd::NLIST * (size_t)max_sym;
   if (glb) B->b.gscr_words = jk::tier_scratch_words(jk::TIER_GLB32) * (size_t)glb_tier_grid(ctx);
   if (hipMalloc(&B->b.bq, sizeof(jk::BQ) * n) != hipSuccess ||
-      hipMalloc(&B->b.bcs, sizeof(int64_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&B->b.bcs, sizeof(int64_t) * 2 * (n + 1)) != hipSuccess ||
       hipMalloc(&B->b.bcnt, sizeof(uint64_t) * n) != hipSuccess ||
       hipMalloc(&B->b.bq32, sizeof(unsigned) * jk::NUM_TIERS * n) != hipSuccess ||
       hipMalloc(&B->b.qctr, 64) != hipSuccess || hipMalloc(&B->b.sscr, sscr * sizeof(int32_t)) != hipSuccess ||
@@ -516,7 +554,7 @@ extern "C" int jgpm_batch_create(jgpm_ctx_t ctx, const jgpm_query* q, int n, jgp
   if (p1_ints && hipMemsetAsync(B->b.p1cnt, 0, p1_ints * sizeof(int32_t), ctx->st) != hipSuccess)
     return fail(JGPM_E_HIP);
   if (hipMemcpyAsync(B->b.bq, B->hq.data(), sizeof(jk::BQ) * n, hipMemcpyHostToDevice, ctx->st) != hipSuccess ||
-      hipMemcpyAsync(B->b.bcs, B->cs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, ctx->st) !=
+      hipMemcpyAsync(B->b.bcs, B->cs.data(), sizeof(int64_t) * 2 * (n + 1), hipMemcpyHostToDevice, ctx->st) !=
           hipSuccess ||
       hipStreamSynchronize(ctx->st) != hipSuccess)
     return fail(JGPM_E_HIP);

@@ -81,6 +81,20 @@ struct P1Args {
 };
 constexpr int P1_WPB = 4;
 hipError_t launch_p1_wedge(const P1Args& a, int grid, hipStream_t st);
+// Every 4-cycle query of a batch (BQ kind 101: rp/ci, first root `base`) in
+// one launch over the flattened roots; p1cs[i] = roots of the 4-cycle
+// queries before query i (nq + 1 entries), counts[i] the query's slot.
+struct P1BatchArgs {
+  const BQ* q;
+  const int64_t* p1cs;
+  int nq;
+  int64_t total;
+  unsigned* qctr;        // zero before the launch
+  int32_t* cnt;          // per wave cnt_stride ints, zero before and after
+  int64_t cnt_stride;    // >= the largest vertex count of the queries
+  uint64_t* counts;
+};
+hipError_t launch_p1_batch(const P1BatchArgs& a, int grid, hipStream_t st);
 
 // One wave per root vertex v_0 of [rb, re) on the symmetric CSR.
 struct PlanArgs {

@@ -104,23 +104,22 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
 //    to c_x, a second takes every counter back to zero with atomicExch and
 //    adds C(c, 2) for the value it took (a repeated x takes 0 after its first
 //    exchange).
-constexpr int P1_LOG_HS = 12;
-constexpr int P1_HS = 1 << P1_LOG_HS;                // LDS slots per wave
 constexpr uint64_t P1_EMPTY = 0xffffffff00000000ull;  // key -1, count 0
 
 // Calls f(x) for every wedge endpoint x of root v0 (its m0 neighbours below
 // v0 start at ci[b0]), 64 neighbours at a time.
 template <class F>
-__device__ void p1_wedges(const P1Args& a, int64_t b0, int m0, int32_t v0, int32_t* off, int64_t* lbase, F f) {
+__device__ void p1_wedges(const int64_t* rp, const int32_t* ci, int64_t b0, int m0, int32_t v0, int32_t* off,
+                          int64_t* lbase, F f) {
   const int ln = lane();
   for (int jb = 0; jb < m0; jb += 64) {
     const int j = jb + ln;
     int len = 0;
     int64_t base = 0;
     if (j < m0) {
-      const int32_t v = a.ci[b0 + j];
-      base = a.rp[v];
-      len = lb(a.ci + base, (int)(a.rp[v + 1] - base), v0);  // x < v0
+      const int32_t v = ci[b0 + j];
+      base = rp[v];
+      len = lb(ci + base, (int)(rp[v + 1] - base), v0);  // x < v0
     }
     int incl = len;
 #pragma unroll
@@ -136,21 +135,84 @@ __device__ void p1_wedges(const P1Args& a, int64_t b0, int m0, int32_t v0, int32
     int i = 0;
     for (int fe = ln; fe < T; fe += 64) {
       i = seg_advance(off, 64, i, fe);
-      f(a.ci[lbase[i] + fe]);
+      f(ci[lbase[i] + fe]);
     }
     __syncwarp();
   }
 }
 
+// The 4-cycle count of root v0: sum over x < v0 of C(c_x, 2), the c_x in the
+// wave's LDS table `tab` (roots with at most P1_HS / 2 wedges) or its dense
+// global counters `cnt` (zero before, left zero).  Wave-uniform result.
+template <int LOG>
+__device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, uint64_t* tab, int32_t* off,
+                            int64_t* lbase, int32_t* cnt) {
+  constexpr int P1_HS = 1 << LOG;
+  constexpr int P1_LOG_HS = LOG;
+  const int ln = lane();
+  const int64_t b0 = rp[v0];
+  const int m0 = lb(ci + b0, (int)(rp[v0 + 1] - b0), v0);  // neighbours below v0
+  // wedge count of the root, to pick the table
+  int64_t w = 0;
+  for (int j = ln; j < m0; j += 64) {
+    const int32_t v = ci[b0 + j];
+    const int64_t base = rp[v];
+    w += lb(ci + base, (int)(rp[v + 1] - base), v0);
+  }
+  w = wsum64(w);
+  uint64_t acc = 0;
+  if (w <= P1_HS / 2) {
+    p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) {
+      const uint64_t key = (uint64_t)(uint32_t)x << 32;
+      int h = (int)(((uint32_t)x * 2654435761u) >> (32 - P1_LOG_HS));
+      for (;;) {
+        uint64_t cur = tab[h];
+        if ((cur & P1_EMPTY) == key) {
+          atomicAdd((unsigned long long*)&tab[h], 1ull);
+          break;
+        }
+        if (cur == P1_EMPTY) {
+          cur = atomicCAS((unsigned long long*)&tab[h], P1_EMPTY, key | 1ull);
+          if (cur == P1_EMPTY) break;
+          if ((cur & P1_EMPTY) == key) {  // claimed by another lane for the same x
+            atomicAdd((unsigned long long*)&tab[h], 1ull);
+            break;
+          }
+        }
+        h = (h + 1) & (P1_HS - 1);
+      }
+    });
+    __syncwarp();
+    for (int k = ln; k < P1_HS; k += 64) {
+      const uint64_t v = tab[k];
+      if (v != P1_EMPTY) {
+        const uint64_t c = (uint32_t)v;
+        acc += c * (c - 1) / 2;
+        tab[k] = P1_EMPTY;
+      }
+    }
+    __syncwarp();
+  } else {
+    p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) { atomicAdd(&cnt[x], 1); });
+    __threadfence();  // the increments are performed before they are taken back
+    p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) {
+      const uint64_t c = (uint32_t)atomicExch(&cnt[x], 0);
+      acc += c * (c - 1) / 2;
+    });
+    __threadfence();
+  }
+  return wave_sum_u64(acc);
+}
+
+template <int LOG>
 __global__ void __launch_bounds__(64 * P1_WPB) p1_wedge_kernel(P1Args a) {
+  constexpr int P1_HS = 1 << LOG;
   __shared__ uint64_t s_tab[P1_WPB][P1_HS];
   __shared__ int32_t s_off[P1_WPB][65];
   __shared__ int64_t s_lb[P1_WPB][64];
   const int wv = threadIdx.x >> 6;
   const int ln = lane();
   uint64_t* tab = s_tab[wv];
-  int32_t* off = s_off[wv];
-  int64_t* lbase = s_lb[wv];
   int32_t* cnt = a.cnt + ((int64_t)blockIdx.x * P1_WPB + wv) * a.nv;
   for (int k = ln; k < P1_HS; k += 64) tab[k] = P1_EMPTY;
   __syncwarp();
@@ -160,64 +222,57 @@ __global__ void __launch_bounds__(64 * P1_WPB) p1_wedge_kernel(P1Args a) {
     if (ln == 0) t = (int64_t)atomicAdd(a.qctr, 1u);
     const int64_t r = a.rb + __shfl(t, 0);
     if (r >= a.re) break;
-    const int32_t v0 = (int32_t)r;
-    const int64_t b0 = a.rp[v0];
-    const int m0 = lb(a.ci + b0, (int)(a.rp[v0 + 1] - b0), v0);  // neighbours below v0
-    // wedge count of the root, to pick the table
-    int64_t w = 0;
-    for (int j = ln; j < m0; j += 64) {
-      const int32_t v = a.ci[b0 + j];
-      const int64_t base = a.rp[v];
-      w += lb(a.ci + base, (int)(a.rp[v + 1] - base), v0);
-    }
-    w = wsum64(w);
-    if (w <= P1_HS / 2) {
-      p1_wedges(a, b0, m0, v0, off, lbase, [&](int32_t x) {
-        const uint64_t key = (uint64_t)(uint32_t)x << 32;
-        int h = (int)(((uint32_t)x * 2654435761u) >> (32 - P1_LOG_HS));
-        for (;;) {
-          uint64_t cur = tab[h];
-          if ((cur & P1_EMPTY) == key) {
-            atomicAdd((unsigned long long*)&tab[h], 1ull);
-            break;
-          }
-          if (cur == P1_EMPTY) {
-            cur = atomicCAS((unsigned long long*)&tab[h], P1_EMPTY, key | 1ull);
-            if (cur == P1_EMPTY) break;
-            if ((cur & P1_EMPTY) == key) {  // claimed by another lane for the same x
-              atomicAdd((unsigned long long*)&tab[h], 1ull);
-              break;
-            }
-          }
-          h = (h + 1) & (P1_HS - 1);
-        }
-      });
-      __syncwarp();
-      for (int k = ln; k < P1_HS; k += 64) {
-        const uint64_t v = tab[k];
-        if (v != P1_EMPTY) {
-          const uint64_t c = (uint32_t)v;
-          acc += c * (c - 1) / 2;
-          tab[k] = P1_EMPTY;
-        }
-      }
-      __syncwarp();
-    } else {
-      p1_wedges(a, b0, m0, v0, off, lbase, [&](int32_t x) { atomicAdd(&cnt[x], 1); });
-      __threadfence();  // the increments are performed before they are taken back
-      p1_wedges(a, b0, m0, v0, off, lbase, [&](int32_t x) {
-        const uint64_t c = (uint32_t)atomicExch(&cnt[x], 0);
-        acc += c * (c - 1) / 2;
-      });
-      __threadfence();
-    }
+    acc += p1_root<LOG>(a.rp, a.ci, (int32_t)r, tab, s_off[wv], s_lb[wv], cnt);
   }
-  acc = wave_sum_u64(acc);
   if (ln == 0 && acc) atomicAdd((unsigned long long*)a.out, (unsigned long long)acc);
 }
 
+// The 4-cycle queries of a batch in one launch: the roots of every query
+// with kind 101 form one flattened task range (prefix p1cs over the query
+// table); a wave takes one root at a time, finds its query by binary search
+// over the prefix, and adds the root's count to the query's slot.
+template <int LOG>
+__global__ void __launch_bounds__(64 * P1_WPB) p1_batch_kernel(P1BatchArgs a) {
+  constexpr int P1_HS = 1 << LOG;
+  __shared__ uint64_t s_tab[P1_WPB][P1_HS];
+  __shared__ int32_t s_off[P1_WPB][65];
+  __shared__ int64_t s_lb[P1_WPB][64];
+  const int wv = threadIdx.x >> 6;
+  const int ln = lane();
+  uint64_t* tab = s_tab[wv];
+  int32_t* cnt = a.cnt + ((int64_t)blockIdx.x * P1_WPB + wv) * a.cnt_stride;
+  for (int k = ln; k < P1_HS; k += 64) tab[k] = P1_EMPTY;
+  __syncwarp();
+  for (;;) {
+    int64_t t = 0;
+    if (ln == 0) t = (int64_t)atomicAdd(a.qctr, 1u);
+    t = __shfl(t, 0);
+    if (t >= a.total) break;
+    int lo = 0, hi = a.nq;  // last query i with p1cs[i] <= t
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.p1cs[mid] <= t) lo = mid; else hi = mid;
+    }
+    const BQ& Q = a.q[lo];
+    const uint64_t c = p1_root<LOG>(Q.rp, Q.ci, (int32_t)(Q.base + (t - a.p1cs[lo])), tab, s_off[wv], s_lb[wv], cnt);
+    if (ln == 0 && c) atomicAdd((unsigned long long*)(a.counts + lo), (unsigned long long)c);
+  }
+}
+
+// log2 of the per-wave LDS table slots: 4096 (32 KiB per wave) measured
+// fastest against 2048 and 1024 on a batch of 256 Erdos-Renyi / Barabasi-
+// Albert 4-cycle queries (323 vs 420 vs 516 ms), though the smaller tables
+// give more waves per CU (R-MAT 18/20 and the 3.8M Chung-Lu graph: within
+// 0-28%, the Chung-Lu graph faster with 1024).
+constexpr int P1_TABLE_LOG2 = 12;
+
+hipError_t launch_p1_batch(const P1BatchArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(p1_batch_kernel<P1_TABLE_LOG2>, dim3(grid), dim3(64 * P1_WPB), 0, st, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_p1_wedge(const P1Args& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(p1_wedge_kernel, dim3(grid), dim3(64 * P1_WPB), 0, st, a);
+  hipLaunchKernelGGL(p1_wedge_kernel<P1_TABLE_LOG2>, dim3(grid), dim3(64 * P1_WPB), 0, st, a);
   return hipGetLastError();
 }
 
