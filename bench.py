@@ -437,6 +437,33 @@ def other_configs(ctx):
     return out
 
 
+def profile_workload(kind):
+    """One pass of the kernels a rocprofv3 summary in profiles/ covers (no
+    timing loop): `sym`: P2, P3, P6, P9 on the 3.8M-vertex Chung-Lu graph and
+    P2, P3, P9 on R-MAT 18 (P6 there costs minutes); `wide`: TC, P13, P14 on
+    R-MAT 24, whose 8,382 roots of DAG out-degree 1025..1775 run in the
+    2048-wide tier."""
+    out = []
+    with J.Context(0) as ctx:
+        if kind == "sym":
+            cl = json.load(open(os.path.join(GOLDEN, "chunglu.json")))
+            g = J.chung_lu(cl["n"], cl["target_edges"], cl["gamma"], cl["max_weight"], seed=cl["seed"])
+            graphs = [("chung-lu 3.8M", ctx.upload(g), ("P2", "P3", "P6", "P9")),
+                      ("R-MAT 18", ctx.rmat_graph(18, EF, 0.57, 0.19, 0.19, seed=SEED), ("P2", "P3", "P9"))]
+            del g
+        else:
+            graphs = [("R-MAT 24", ctx.rmat_graph(24, EF, 0.57, 0.19, 0.19, seed=SEED), ("TC", "P13", "P14"))]
+        for name, G, pats in graphs:
+            for p in pats:
+                G.count(p, (0, 1))  # builds the per-graph arrays outside the profiled pass
+                c, ms = G.count(p)
+                out.append({"graph": name, "pattern": p, "count": c, "kernel_ms": round(ms, 3),
+                            "kernels": ctx.last_profile()})
+                log(f"{name} {p}: {c} ({ms:.2f} ms)")
+            G.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -448,7 +475,8 @@ def main():
     ap.add_argument("--scale", type=int, default=SCALE)
     ap.add_argument("--profile", action="store_true",
                     help="only the counting kernels of one step (for rocprofv3 counter runs)")
-    ap.add_argument("--workload", choices=["cliques", "batch", "serve"], default="cliques")
+    ap.add_argument("--workload", choices=["cliques", "batch", "serve", "profile-sym", "profile-wide"],
+                    default="cliques")
     ap.add_argument("--batch-graphs", type=int, default=2048)
     ap.add_argument("--batch-max-n", type=int, default=262144)
     ap.add_argument("--serve-min-scale", type=int, default=18)
@@ -472,6 +500,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         print(json.dumps(workload_batch(args)), flush=True)
+        return
+    if args.workload.startswith("profile-"):
+        print(json.dumps(profile_workload(args.workload[len("profile-"):])), flush=True)
         return
     if args.workload == "serve":
         with J.Context(0) as ctx:
