@@ -45,6 +45,31 @@ struct Wave {
   int32_t* stage;  // LDS, STAGE ints
 };
 
+// |A ∩ B| of two long sorted lists of comparable length (na <= nb <=
+// MERGE_RATIO * na, na beyond the LDS staging size): a chunked merge.  Lane l
+// takes the l-th of 64 contiguous chunks of A, finds where the chunk's first
+// element falls in B by one binary search, then walks both lists forward as a
+// sequential merge; every element of A is counted once, by the lane that owns
+// it, and the B spans of neighbouring lanes meet at the chunk boundaries, so
+// the wave reads about na + nb elements instead of na searches of log2(nb)
+// dependent loads each (the merge side of the degree-ratio switch; the
+// reference's CPU hybrid switches at ratio 32/50, common/intersect.cc:11,54).
+constexpr int MERGE_RATIO = 8;
+__device__ int64_t w_merge_count(const int32_t* A, int na, const int32_t* B, int nb) {
+  const int chunk = (na + 63) >> 6;
+  const int a0 = min(na, lane() * chunk), a1 = min(na, a0 + chunk);
+  int c = 0;
+  if (a0 < a1) {
+    int j = lb(B, nb, A[a0]);
+    for (int i = a0; i < a1 && j < nb; ++i) {
+      const int32_t x = A[i];
+      while (j < nb && B[j] < x) ++j;
+      c += (j < nb && B[j] == x);
+    }
+  }
+  return wsum(c);
+}
+
 // |A ∩ B| restricted to values < ub.  All lanes call with uniform arguments.
 __device__ int64_t w_icount(Wave& w, const int32_t* A, int na, const int32_t* B, int nb, int32_t ub) {
   if (ub != INT32_MAX) {
@@ -56,6 +81,7 @@ __device__ int64_t w_icount(Wave& w, const int32_t* A, int na, const int32_t* B,
     int u = na; na = nb; nb = u;
   }
   if (na == 0) return 0;
+  if (na > STAGE && nb <= MERGE_RATIO * na) return w_merge_count(A, na, B, nb);
   const int32_t* S = B;
   if (nb <= STAGE && na > 4) {
     for (int k = lane(); k < nb; k += 64) w.stage[k] = B[k];
