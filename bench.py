@@ -21,6 +21,9 @@ Also on the same JSON line:
   other_configs the triangle count on R-MAT 14 (launch-bound latency) and
                 the vertex-induced 4-vertex motifs on the 3.8M-vertex
                 Chung-Lu graph, each checked against frozen oracle counts;
+  batched       the 2,048-graph batch (half Erdos-Renyi, half Barabasi-
+                Albert, one pattern each from {TC, P13, P1, P2}): autotuned,
+                captured and replayed vs one plain launch per query;
   serving       Poisson request streams over 32 resident R-MAT graphs of
                 scales 18-24 (uniform graph, uniform pattern id P1..P16 within
                 the stated per-pattern caps), continuous batching vs the
@@ -213,15 +216,19 @@ def config4_graph(spec):
     return J.erdos_renyi(n, par, seed=sd) if kind == "ER" else J.barabasi_albert(n, par, seed=sd)
 
 
-def workload_batch(args):
+def workload_batch(args, ctx=None):
     """Many independent graphs, each paired with one pattern from
     {TC, P13, P1, P2}: the batch autotuned, captured once in a HIP graph and
     replayed, vs one plain launch per query.  The default is the full
-    configuration (2,048 graphs, n up to 262,144)."""
+    configuration (2,048 graphs, n up to 262,144); its counts are checked
+    against the plain launches and a sample against the oracle's frozen
+    counts."""
     specs = config4_specs(args.batch_graphs, args.batch_max_n)
     t0 = time.time()
     edges = 0
-    ctx = J.Context(0)
+    own = ctx is None
+    if own:
+        ctx = J.Context(0)
     Gs = []
     for i, spec in enumerate(specs):
         c = config4_graph(spec)
@@ -229,8 +236,9 @@ def workload_batch(args):
         Gs.append(ctx.upload(c))  # resident; the host copy is dropped
         if (i + 1) % 128 == 0:
             log(f"generated and uploaded {i + 1} graphs, {edges} edges ({time.time() - t0:.1f}s)")
-    log(f"generated {len(Gs)} graphs, {edges} edges ({time.time() - t0:.1f}s)")
-    with ctx:
+    gen_s = time.time() - t0
+    log(f"generated {len(Gs)} graphs, {edges} edges ({gen_s:.1f}s)")
+    try:
         queries = [(G, s[4], 0, None) for G, s in zip(Gs, specs)]
         for _ in range(args.warmup):
             ctx.count_batch(queries)
@@ -263,8 +271,17 @@ def workload_batch(args):
             if got != plain_counts:
                 bad = [i for i, (x, y) in enumerate(zip(got, plain_counts)) if x != y]
                 raise AssertionError(f"{name} counts differ from plain counts at {bad[:10]}")
+        oracle_checked = 0
+        if args.batch_graphs == 2048 and args.batch_max_n == 262144:
+            for ent in json.load(open(os.path.join(GOLDEN, "batch2048.json")))["sample"]:
+                if plain_counts[ent["index"]] != ent["count"]:
+                    raise AssertionError(f"batch graph {ent['index']}: {plain_counts[ent['index']]} != oracle {ent['count']}")
+                oracle_checked += 1
+    finally:
         for G in Gs:
             G.free()
+        if own:
+            ctx.close()
     nq = len(queries)
     knobs = {k: tuned[k] for k in ("batch_hub_deg", "batch_root_chunk", "batch_edge_chunk", "batch_blocks_per_cu")}
     return {"metric": METRIC, "value": round(nq / t_batch, 3), "unit": "queries/s", "n_gpus": 1,
@@ -278,7 +295,8 @@ def workload_batch(args):
                        "eager_batched_queries_per_s": round(nq / t_eager, 3),
                        "plain_queries_per_s": round(nq / t_plain, 3),
                        "plain_kernel_ms_by_pattern": {p: round(v, 1) for p, v in sorted(plain_ms.items())},
-                       "speedup_vs_plain": round(t_plain / t_batch, 3), "counts_equal_plain": True}}
+                       "speedup_vs_plain": round(t_plain / t_batch, 3), "counts_equal_plain": True,
+                       "counts_checked_against_oracle": oracle_checked, "host_build_and_upload_s": round(gen_s, 1)}}
 
 
 # Largest R-MAT scale a served pattern draws its graph from (the request's
@@ -472,6 +490,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-other", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--scale", type=int, default=SCALE)
     ap.add_argument("--profile", action="store_true",
                     help="only the counting kernels of one step (for rocprofv3 counter runs)")
@@ -603,6 +622,10 @@ def main():
         if not args.no_other:
             log("other configurations ...")
             line["other_configs"] = other_configs(ctx)
+        if not args.no_batch:
+            log("batched workload ...")
+            b = workload_batch(argparse.Namespace(**{**vars(args), "steps": 3, "warmup": 1}), ctx)
+            line["batched"] = {"queries_per_s": b["value"], "ms_per_step": b["ms_per_step"], **b["config"]}
         if not args.no_serve:
             log("serving ...")
             line["serving"] = serving(ctx, args.rates, args.requests, args.serve_min_scale, args.serve_max_scale)
