@@ -14,6 +14,11 @@
 #include <new>
 #include <string>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "jgpm_internal.hpp"
 
 namespace jgpm {
@@ -166,23 +171,20 @@ int write_csr(const jgpm_csr_s& c, const char* prefix) {
   return JGPM_OK;
 }
 
-int check_csr(const jgpm_csr_s& c, bool symmetric, int bad) {
-  const int64_t nv = c.nv;
-  if ((int64_t)c.rowptr.size() != nv + 1 || (int64_t)c.col.size() != c.nnz) return bad;
-  if (c.rowptr[0] != 0 || c.rowptr[nv] != c.nnz) return bad;
+int check_csr_arrays(int64_t nv, int64_t nnz, const int64_t* rp, const int32_t* col, bool symmetric, int bad) {
+  if (rp[0] != 0 || rp[nv] != nnz) return bad;
   for (int64_t v = 0; v < nv; ++v)
-    if (c.rowptr[v + 1] < c.rowptr[v]) return bad;
+    if (rp[v + 1] < rp[v]) return bad;
   std::atomic<bool> ok{true};
   parallel_for(nv, 1024, [&](int64_t b, int64_t e) {
     for (int64_t v = b; v < e && ok.load(std::memory_order_relaxed); ++v) {
-      const int32_t* l = c.col.data() + c.rowptr[v];
-      const int64_t d = c.rowptr[v + 1] - c.rowptr[v];
+      const int32_t* l = col + rp[v];
+      const int64_t d = rp[v + 1] - rp[v];
       for (int64_t k = 0; k < d; ++k) {
         const int32_t u = l[k];
         if (u < 0 || u >= nv || u == v || (k > 0 && u <= l[k - 1])) { ok = false; break; }
         if (symmetric && u > v) {
-          const int32_t* m = c.col.data() + c.rowptr[u];
-          if (!std::binary_search(m, c.col.data() + c.rowptr[u + 1], (int32_t)v)) { ok = false; break; }
+          if (!std::binary_search(col + rp[u], col + rp[u + 1], (int32_t)v)) { ok = false; break; }
         }
       }
     }
@@ -195,13 +197,66 @@ int check_csr(const jgpm_csr_s& c, bool symmetric, int bad) {
     parallel_for(nv, 4096, [&](int64_t b, int64_t e) {
       int64_t a = 0, z = 0;
       for (int64_t v = b; v < e; ++v)
-        for (int64_t k = c.rowptr[v]; k < c.rowptr[v + 1]; ++k) (c.col[k] < v ? a : z) += 1;
+        for (int64_t k = rp[v]; k < rp[v + 1]; ++k) (col[k] < v ? a : z) += 1;
       lo += a;
       hi += z;
     });
     if (lo.load() != hi.load()) return bad;
   }
   return JGPM_OK;
+}
+
+int check_csr(const jgpm_csr_s& c, bool symmetric, int bad) {
+  if ((int64_t)c.rowptr.size() != (int64_t)c.nv + 1 || (int64_t)c.col.size() != c.nnz) return bad;
+  return check_csr_arrays(c.nv, c.nnz, c.rowptr.data(), c.col.data(), symmetric, bad);
+}
+
+// The reference's files mapped read-only (include/custom_alloc.h maps them
+// the same way for out-of-core graphs): no copy into host vectors, the pages
+// are read once by the checks and once by the upload.
+int map_csr(const char* prefix, MappedCsr& m) {
+  std::string p(prefix);
+  std::ifstream fm(p + ".meta.txt");
+  if (!fm) return JGPM_E_IO;
+  int64_t nv = 0, ne = 0;
+  int vid_size = 0, eid_size = 0, vl = 0, el = 0, md = 0;
+  fm >> nv >> ne >> vid_size >> eid_size >> vl >> el >> md;
+  if (!fm || vid_size != 4 || eid_size != 8 || nv <= 0 || nv > INT32_MAX || ne < 0) return JGPM_E_IO;
+  auto map_file = [](const std::string& f, size_t want, void*& base, size_t& len) -> int {
+    const int fd = open(f.c_str(), O_RDONLY);
+    if (fd < 0) return JGPM_E_IO;
+    struct stat st;
+    if (fstat(fd, &st) != 0 || (size_t)st.st_size < want) {
+      close(fd);
+      return JGPM_E_IO;
+    }
+    len = std::max<size_t>(want, 1);
+    base = want ? mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0) : nullptr;
+    close(fd);
+    if (want && base == MAP_FAILED) {
+      base = nullptr;
+      return JGPM_E_IO;
+    }
+    return JGPM_OK;
+  };
+  int st = map_file(p + ".vertex.bin", sizeof(int64_t) * (size_t)(nv + 1), m.map_v, m.len_v);
+  if (!st) st = map_file(p + ".edge.bin", sizeof(int32_t) * (size_t)ne, m.map_e, m.len_e);
+  if (st) {
+    unmap_csr(m);
+    return st;
+  }
+  m.nv = (int32_t)nv;
+  m.nnz = ne;
+  m.rp = static_cast<const int64_t*>(m.map_v);
+  static const int32_t empty = 0;
+  m.ci = ne ? static_cast<const int32_t*>(m.map_e) : &empty;
+  return JGPM_OK;
+}
+
+void unmap_csr(MappedCsr& m) {
+  if (m.map_v) munmap(m.map_v, m.len_v);
+  if (m.map_e) munmap(m.map_e, m.len_e);
+  m = MappedCsr{};
 }
 
 int read_csr(const char* prefix, jgpm_csr_s& c) {

@@ -139,6 +139,9 @@ int scan_i64(Scratch& S, const int64_t* in, int64_t* out, int64_t n, hipStream_t
   return JGPM_OK;
 }
 
+int orient_on_device(jgpm_ctx_s* c, Scratch& S, jgpm_graph_s* g, int32_t* row, int32_t nv, int64_t nnz,
+                     bool dag_only);
+
 // Edge list already on the device (d_src, d_dst, m pairs) -> graph handle.
 int build_on_device(jgpm_ctx_s* c, Scratch& S, int32_t* d_src, int32_t* d_dst, int64_t m, int32_t nv,
                     jgpm_graph_t* out) {
@@ -200,6 +203,26 @@ int build_on_device(jgpm_ctx_s* c, Scratch& S, int32_t* d_src, int32_t* d_dst, i
   if (rc) return fail(rc);
   S.drop(keys);  // both sort buffers (the keys and the keep flags) are done
   S.drop(alt);
+  S.drop(pos);
+  S.drop(deg);
+  rc = orient_on_device(c, S, g, row, nv, nnz, false);
+  if (rc) return fail(rc);
+  *out = g;
+  return JGPM_OK;
+}
+
+// Steps 4-5 for a graph whose symmetric CSR (g->sym.rp / ci, nv rows, nnz
+// entries, ascending duplicate-free lists) is on the device, with row[e] the
+// source of entry e: the degree DAG, compacted in order, then the host
+// metadata.  dag_only frees the symmetric arrays afterwards (a DAG-only
+// handle, as Graph(prefix, use_dag = true)).
+int orient_on_device(jgpm_ctx_s* c, Scratch& S, jgpm_graph_s* g, int32_t* row, int32_t nv, int64_t nnz,
+                     bool dag_only) {
+  hipStream_t st = c->st;
+  auto fail = [&](int code) { return code; };
+  int rc = JGPM_OK;
+  unsigned long long* deg = S.get<unsigned long long>((size_t)nv + 1);
+  if (!deg) return JGPM_E_NOMEM;
   // 4: degree DAG
   int64_t* dkeep = S.get<int64_t>(nnz + 1);
   int64_t* dpos = S.get<int64_t>(nnz + 1);
@@ -251,7 +274,15 @@ int build_on_device(jgpm_ctx_s* c, Scratch& S, int32_t* d_src, int32_t* d_dst, i
   rc = jgpm::build_tiers(g, g->dag_rp_h.data(), nv);
   if (!rc) rc = jgpm::set_degree_order(g, g->sym_rp_h, smax);
   if (rc) return fail(rc);
-  *out = g;
+  if (dag_only) {
+    if (hipStreamSynchronize(st) != hipSuccess) return JGPM_E_HIP;
+    (void)hipFree(g->sym.rp);
+    (void)hipFree(g->sym.ci);
+    g->sym = DevCSR{};
+    g->has_sym = false;
+    g->sym_rp_h.clear();
+    g->bytes = 8 * ((int64_t)nv + 1) + 4 * dnnz;
+  }
   return JGPM_OK;
 }
 
@@ -311,6 +342,48 @@ extern "C" int jgpm_graph_rmat(jgpm_ctx_t ctx, int scale, int edge_factor, doubl
   if (hipGetLastError() != hipSuccess) return JGPM_E_HIP;
   return build_on_device(ctx, S, ds, dd, m, nv, out);
 }
+
+namespace {
+__global__ void k_rows(const int64_t* rp, int32_t nv, int32_t* row) {
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int ln = threadIdx.x & 63;
+  for (int64_t v = wid; v < nv; v += nw)
+    for (int64_t e = rp[v] + ln; e < rp[v + 1]; e += 64) row[e] = (int32_t)v;
+}
+}  // namespace
+
+namespace jgpm {
+// A checked symmetric CSR in host memory (a loaded file, possibly mapped)
+// -> resident graph, the degree DAG built on the device.
+int graph_from_host_sym(jgpm_ctx_s* c, const int64_t* rowptr, const int32_t* colidx, int32_t nv, int64_t nnz,
+                        bool dag_only, jgpm_graph_t* out) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->dev) != hipSuccess) return JGPM_E_HIP;
+  Scratch S;
+  S.st = c->st;
+  auto* g = new (std::nothrow) jgpm_graph_s();
+  if (!g) return JGPM_E_NOMEM;
+  g->ctx = c;
+  auto fail = [&](int code) {
+    jgpm_graph_free(g);
+    return code;
+  };
+  int32_t* row = S.get<int32_t>(nnz);
+  if (!row || hipMalloc(&g->sym.ci, sizeof(int32_t) * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+      hipMalloc(&g->sym.rp, sizeof(int64_t) * ((size_t)nv + 1)) != hipSuccess)
+    return fail(JGPM_E_NOMEM);
+  if (hipMemcpy(g->sym.rp, rowptr, 8 * ((size_t)nv + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      (nnz && hipMemcpy(g->sym.ci, colidx, 4 * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess))
+    return fail(JGPM_E_HIP);
+  hipLaunchKernelGGL(k_rows, dim3(grid_for((int64_t)nv * 64)), dim3(256), 0, c->st, g->sym.rp, nv, row);
+  if (hipGetLastError() != hipSuccess) return fail(JGPM_E_HIP);
+  const int rc = orient_on_device(c, S, g, row, nv, nnz, dag_only);
+  if (rc) return fail(rc);
+  *out = g;
+  return JGPM_OK;
+}
+}  // namespace jgpm
 
 extern "C" int jgpm_graph_download(jgpm_graph_t g, int dag, jgpm_csr_t* out) {
   if (!g || !out) return JGPM_E_INVALID;

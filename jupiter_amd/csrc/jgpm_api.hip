@@ -372,18 +372,16 @@ extern "C" int jgpm_graph_upload(jgpm_ctx_t c, const int64_t* rowptr, const int3
 
 extern "C" int jgpm_graph_load(jgpm_ctx_t c, const char* prefix, int dag_only, jgpm_graph_t* out) {
   if (!c || !prefix || !out) return JGPM_E_INVALID;
-  jgpm_csr_s h;
-  int st = read_csr(prefix, h);
+  // the files are mapped, checked where they lie (the file must hold the
+  // symmetric graph: the DAG is derived from it), uploaded, and oriented on
+  // the device
+  MappedCsr m;
+  int st = map_csr(prefix, m);
   if (st) return st;
-  // the file must hold the symmetric graph: the DAG is derived from it
-  if ((st = check_csr(h, true, JGPM_E_IO))) return st;
-  jgpm_csr_s dag;
-  try {
-    orient_dag(h, dag);
-  } catch (const std::bad_alloc&) {
-    return JGPM_E_NOMEM;
-  }
-  return graph_from_host(c, dag_only ? nullptr : &h, &dag, out);
+  st = check_csr_arrays(m.nv, m.nnz, m.rp, m.ci, true, JGPM_E_IO);
+  if (!st) st = graph_from_host_sym(c, m.rp, m.ci, m.nv, m.nnz, dag_only != 0, out);
+  unmap_csr(m);
+  return st;
 }
 
 extern "C" int jgpm_graph_free(jgpm_graph_t g) {

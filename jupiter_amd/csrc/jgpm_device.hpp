@@ -91,6 +91,10 @@ size_t p1_counter_ints(jgpm_ctx_s* c, int32_t nv);
 // g->rsym: vertex v becomes d_ord[v] (degree order), lists re-sorted.  Built
 // once per graph (allocates; never while capturing).
 int ensure_rank_csr(jgpm_graph_s* g, hipStream_t st);
+// A checked symmetric CSR in host memory -> resident graph; the degree DAG is
+// built on the device (ingest.hip).  dag_only keeps only the DAG.
+int graph_from_host_sym(jgpm_ctx_s* c, const int64_t* rowptr, const int32_t* colidx, int32_t nv, int64_t nnz,
+                        bool dag_only, jgpm_graph_t* out);
 int ensure_p1_counters(jgpm_ctx_s* c, size_t need, hipStream_t st);
 int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, int32_t* cnt, size_t cnt_ints,
               hipStream_t st);

@@ -71,6 +71,20 @@ int build_sym_csr(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
 // duplicates) without self loops; with `symmetric`, v in N(u) iff u in N(v).
 // Returns JGPM_OK or `bad` (the caller's status for a malformed graph).
 int check_csr(const jgpm_csr_s& c, bool symmetric, int bad);
+int check_csr_arrays(int64_t nv, int64_t nnz, const int64_t* rp, const int32_t* col, bool symmetric, int bad);
+// <prefix>.vertex.bin / .edge.bin mapped read-only (meta checked); unmap_csr
+// releases them.
+struct MappedCsr {
+  int32_t nv = 0;
+  int64_t nnz = 0;
+  const int64_t* rp = nullptr;
+  const int32_t* ci = nullptr;
+  void* map_v = nullptr;
+  void* map_e = nullptr;
+  size_t len_v = 0, len_e = 0;
+};
+int map_csr(const char* prefix, MappedCsr& m);
+void unmap_csr(MappedCsr& m);
 void orient_dag(const jgpm_csr_s& sym, jgpm_csr_s& dag);
 int32_t compute_max_degree(const jgpm_csr_s& c);
 int write_csr(const jgpm_csr_s& c, const char* prefix);

@@ -65,3 +65,35 @@ def test_device_edges_edge_cases(ctx):
     with pytest.raises(J.JgpmError) as e:
         ctx.graph_from_edges(4, [0, 5], [1, 2])  # id out of range
     assert e.value.status == -1
+
+
+def test_file_load_orients_on_the_device(ctx, tmp_path):
+    """jgpm_graph_load maps the reference's files, checks them in place and
+    builds the degree DAG on the device: the arrays equal the host
+    construction, counts equal an upload, and a DAG-only load keeps the DAG
+    alone."""
+    host = J.rmat(14, 16, seed=4)
+    prefix = str(tmp_path / "g")
+    host.write(prefix)
+    _say("file load, symmetric + DAG")
+    G = ctx.load(prefix)
+    U = ctx.upload(host)
+    try:
+        _same(G.download(), host)
+        _same(G.download(dag=True), host.orient())
+        for p in ("TC", "P13", "P1", "P2"):
+            assert G.count(p)[0] == U.count(p)[0], p
+    finally:
+        G.free()
+        U.free()
+    _say("file load, DAG only")
+    D = ctx.load(prefix, dag_only=True)
+    try:
+        _same(D.download(dag=True), host.orient())
+        assert D.info()["sym_nnz"] == -1
+        with pytest.raises(J.JgpmError) as e:
+            D.count("P2")
+        assert e.value.status == -7
+        assert D.count("P14")[0] > 0
+    finally:
+        D.free()
