@@ -157,21 +157,23 @@ def cpu_baseline(dag, roots_end, seed=7, per_stratum=(256, 64, 32, 16, 8), batch
             O.count_roots(dag.rowptr, dag.col, J.PATTERNS[p], roots, mode=1, threads=nthreads)
         return time.perf_counter() - t0
 
-    est1, var1, secs1, light1 = 0.0, 0.0, 0.0, 0.0
-    for k, (n_s, take) in enumerate(strata):
+    est1, var1, secs1 = 0.0, 0.0, 0.0
+    for n_s, take in strata:
         means = []
         for part in np.array_split(rng.permutation(take), batches):
             dt = timed(np.sort(part), 1)
             secs1 += dt
-            light1 += dt if k < 2 else 0.0
             means.append(dt / max(1, len(part)))
         means = np.array(means)
         est1 += n_s * means.mean()
         var1 += (n_s ** 2) * means.var(ddof=1) / len(means)
-    # parallel speedup on the two light strata, whose many small roots keep
-    # every thread busy as the full query's 10^6 roots do (the few sampled
-    # hub roots alone cannot: the slowest of them bounds a parallel call)
-    light = np.sort(np.concatenate([t for _, t in strata[:2]]))
+    # parallel speedup on 16,384 roots of the two light strata, whose many
+    # small roots keep every thread busy as the full query's 10^6 roots do
+    # (the few sampled hub roots alone cannot: the slowest of them bounds a
+    # parallel call)
+    pool = np.nonzero(d <= 128)[0]
+    light = np.sort(rng.choice(pool, size=min(16384, len(pool)), replace=False)).astype(np.int32)
+    light1 = timed(light, 1)
     wall = timed(light, threads)
     speedup = light1 / wall
     se1 = float(np.sqrt(var1))
