@@ -161,10 +161,16 @@ __device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, ui
   }
   w = wsum64(w);
   uint64_t acc = 0;
+  if (w < 2) return 0;  // no pair of wedges shares an endpoint
   if (w <= P1_HS / 2) {
+    // only the first 2^lt slots (load <= 1/2) are used, so the sweep below
+    // costs the root's own wedge count, not the whole table
+    int lt = 6;
+    while (lt < P1_LOG_HS && ((int64_t)1 << lt) < 2 * w) ++lt;
+    const int used = 1 << lt;
     p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) {
       const uint64_t key = (uint64_t)(uint32_t)x << 32;
-      int h = (int)(((uint32_t)x * 2654435761u) >> (32 - P1_LOG_HS));
+      int h = (int)(((uint32_t)x * 2654435761u) >> (32 - lt));
       for (;;) {
         uint64_t cur = tab[h];
         if ((cur & P1_EMPTY) == key) {
@@ -179,11 +185,11 @@ __device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, ui
             break;
           }
         }
-        h = (h + 1) & (P1_HS - 1);
+        h = (h + 1) & (used - 1);
       }
     });
     __syncwarp();
-    for (int k = ln; k < P1_HS; k += 64) {
+    for (int k = ln; k < used; k += 64) {
       const uint64_t v = tab[k];
       if (v != P1_EMPTY) {
         const uint64_t c = (uint32_t)v;
