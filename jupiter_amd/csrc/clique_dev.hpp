@@ -462,9 +462,10 @@ __device__ __forceinline__ uint32_t dot16(const v16f32& c, const v16f32& m) {
 // operand, so the accumulators need no clearing.  J lists S's positions in
 // ascending order (n > 0 entries); every lane of the wave must call this.
 // This form keeps one accumulator per block pair and reads the edge mask from
-// the rows (narrow roots, where registers decide occupancy).
+// the rows, bit by bit (the 256-wide roots: rows of 4 words, where the
+// members of a block mostly share a word and registers decide occupancy).
 template <class RowT>
-__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int whi_, int zr_) {
+__device__ uint64_t tri_mfma1_bits(const RowT& R, const int16_t* J, int n_, int i_, int whi_, int zr_) {
   const int n = uni(n_), i = uni(i_), whi = uni(whi_), zr = uni(zr_);
   const int ln = lane_id();
   const int col = ln & 31, half = ln >> 5;
@@ -500,6 +501,76 @@ __device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, i
       }
       if (q <= whi) acc = step(q, acc, vb, va);
       s += edge_masked_sum(R, J, n, whi, Y, x, half, acc);
+    }
+  }
+  return s;
+}
+
+// Triangles of the DAG induced on S = R[i] (rows ranked in DAG order, so
+// every row only has bits above its own position), on the matrix cores:
+//   sum over member pairs x -> y of |R[y] ∩ R[i] ∩ R[x]|
+// = sum over 32x32 blocks (X <= Y) of  mask(x -> y) . (A_Y B_X^T)[y][x],
+// where A_Y holds the rows R[y] ∩ R[i] of the Y block's members and B_X the
+// rows R[x] of the X block's members, over the row's words from the one
+// holding the block's first member to whi, one fp4 MFMA per 64-position
+// word.  The mask is the product E_Y B_X^T of rows one-hot at the Y members'
+// own positions, accumulated on the words that hold them (qa .. qb) with the
+// same B operands: its entry [y][x] is bit y of R[x], the edge x -> y.
+// Lane l supplies member 32B + l%32 and the 32 positions 32 * (l / 32) .. +31
+// of the word; accumulator element a of lane l is entry
+// [8 (a / 4) + 4 (l / 32) + a % 4][l % 32].  Absent members (past n) read row
+// zr, which is empty (the last row of the DAG order), so every load and
+// product is unconditional; the first word's products start from a zero C
+// operand, so the accumulators need no clearing.  J lists S's positions in
+// ascending order (n > 0 entries); every lane of the wave must call this.
+// This form keeps one block pair at a time (the 512-wide roots, where
+// registers decide occupancy); on rows of 512 positions it replaces sixteen
+// bit tests per lane over up to eight words (tri_mfma1_bits) with one more
+// product per word that holds a member of the block (P14 on R-MAT 22, roots
+// [0, 2^20): 512-wide tier 21.5 -> 14.3 ms).
+template <class RowT>
+__device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int whi_, int zr_) {
+  const int n = uni(n_), i = uni(i_), whi = uni(whi_), zr = uni(zr_);
+  const int ln = lane_id();
+  const int col = ln & 31, half = ln >> 5;
+  const int T = (n + 31) >> 5;
+  uint64_t s = 0;
+  for (int Y = 0; Y < T; ++Y) {
+    const int ay = 32 * Y + col;
+    const int y = ay < n ? (int)J[ay] : -1;
+    // words qa .. qb hold the block's members (qb <= whi: S lies in row i)
+    const int qa = uni(J[32 * Y]) >> 6, qb = uni(J[min(n - 1, 32 * Y + 31)]) >> 6;
+    const int ry = y >= 0 ? y : zr;
+    // the one-hot row of member y: bit y & 31 of half (y >> 5) & 1 of word y >> 6
+    const uint32_t hbit = (y >= 0 && ((y >> 5) & 1) == half) ? 1u << (y & 31) : 0u;
+    const int hq = y >= 0 ? y >> 6 : -1;
+    for (int X = 0; X <= Y; ++X) {
+      const int bx = 32 * X + col;
+      const int rx = bx < n ? (int)J[bx] : zr;
+      // word q + 1's row halves are loaded before word q's table reads
+      // (two register sets alternate, so the loop moves no values)
+      const RowCur ki = R.cur(i, half), ky = R.cur(ry, half), kx = R.cur(rx, half);
+      uint32_t va[3] = {R.at(ki, qa), R.at(ky, qa), R.at(kx, qa)}, vb[3];
+      v16f32 acc, msk;
+      auto step = [&](int q, bool first, bool with_mask, const uint32_t(&v)[3], uint32_t(&nx)[3]) {
+        const uint32_t sa = v[1] & v[0], sb = v[2];
+        const int wn = min(q + 1, whi);
+        nx[0] = R.at(ki, wn);
+        nx[1] = R.at(ky, wn);
+        nx[2] = R.at(kx, wn);
+        const v8i32 b = enc_b(sb);
+        const v16f32 z = zero16();
+        acc = mma_fp4(enc_a(sa), b, first ? z : acc);
+        if (with_mask) msk = mma_fp4(enc_a(hq == q ? hbit : 0u), b, first ? z : msk);
+      };
+      step(qa, true, true, va, vb);
+      int q = qa + 1;
+      for (; q + 1 <= whi; q += 2) {
+        step(q, false, q <= qb, vb, va);
+        step(q + 1, false, q + 1 <= qb, va, vb);
+      }
+      if (q <= whi) step(q, false, q <= qb, vb, va);
+      s += dot16(acc, msk);
     }
   }
   return s;
@@ -649,9 +720,6 @@ struct Lay {
   static constexpr int DM = 64 * WMAX;
   static constexpr bool ROWS = K >= 4;
   static constexpr bool MULTI = WMAX > 1 && ROWS;
-  // compressed-row capacity of the block-cooperative path (rows of S > 64)
-  static constexpr int NC = 0;  // (no block-cooperative compressed rows)
-  static constexpr int CWMAX = NC > 0 ? NC / 64 : 1;
   static constexpr size_t al(size_t x) { return (x + 15) & ~size_t(15); }
   static constexpr size_t L_OFF = 0;
   static constexpr size_t W_OFF = al(L_OFF + (BUILD ? 4 * DM : 0));  // the root list (build only)
@@ -670,13 +738,11 @@ struct Lay {
   static constexpr size_t B_KEY = al(B_HFL + 4 * ((DM + 31) / 32));      // int32 [DM] order keys, then int16 [HCAP] hash
   static constexpr size_t B_RNK = al(B_KEY + (HBYTES > KBYTES ? HBYTES : KBYTES));  // int16 [DM] rank in DAG order
   static constexpr size_t B_END = al(B_RNK + (MULTI ? 2 * DM : 0));
-  // count view
-  static constexpr size_t C_J = 0;                                          // int16 [G][DM]
-  static constexpr size_t C_COMP = al(C_J + (MULTI ? 2 * G * DM : 0));     // u64 [NC * CWMAX]
-  static constexpr size_t C_SC = al(C_COMP + (size_t)8 * NC * CWMAX);       // u64 [G][64]
-  static constexpr size_t C_SCW = al(C_SC + (MULTI ? (size_t)8 * G * 64 : 0));  // u32 [G][64]
-  static constexpr size_t C_WCB = C_SCW;
-  static constexpr size_t C_END = al(C_WCB + ((MULTI && BYTES) ? (size_t)4 * NC : 0));
+  // count view, one slot of C_PW bytes per wave: the positions J of S
+  // (int16 [DM]) and the successor bitmap Rc (u64 [64])
+  static constexpr size_t C_RC = al(2 * (size_t)DM);
+  static constexpr size_t C_PW = MULTI ? C_RC + 8 * 64 : 0;
+  static constexpr size_t C_END = al((size_t)G * C_PW);
   static constexpr size_t U_SIZE = (!CNT || B_END > C_END) ? B_END : C_END;  // CNT: the count view is used
   static constexpr size_t SLOT = al(U_OFF + U_SIZE);
 };
@@ -899,8 +965,9 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     if constexpr (K == 4 && !BYTES && WMAX >= 8) {
       acc += tri4_mfma(R, d, W, wv, G);
     } else {
-    int16_t* J = reinterpret_cast<int16_t*>(U + LY::C_J) + (size_t)wv * DM;
-    uint64_t* Rc = reinterpret_cast<uint64_t*>(U + LY::C_SC) + (size_t)wv * 64;
+    unsigned char* slot = U + (size_t)wv * LY::C_PW;
+    int16_t* J = reinterpret_cast<int16_t*>(slot);
+    uint64_t* Rc = reinterpret_cast<uint64_t*>(slot + LY::C_RC);
     // waves take rows in order from a shared counter (s_wtot[G], zeroed
     // before the group's last barrier): early rows in DAG order hold the most
     // work, so a fixed interleave leaves waves idle at the end.  4-cliques
@@ -974,9 +1041,11 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         }
       } else {
         if constexpr (MMA5) {
-          // tiles of block pairs pay for their registers only on the widest roots
+          // tiles of block pairs pay for their registers only on the widest
+          // roots; the edge mask is a product from 512 wide up
           if constexpr (WMAX >= 16) acc += tri_mfma(R, J, n, i, whi, d - 1);
-          else acc += tri_mfma1(R, J, n, i, whi, d - 1);
+          else if constexpr (WMAX >= 8) acc += tri_mfma1(R, J, n, i, whi, d - 1);
+          else acc += tri_mfma1_bits(R, J, n, i, whi, d - 1);
         } else if constexpr (K == 5 && WMAX <= 16) {
           // 5-cliques: triangles inside S.  One flat loop per lane over a
           // work list shared through an LDS counter: a step either takes the
