@@ -523,11 +523,13 @@ __device__ uint64_t tri_mfma1_bits(const RowT& R, const int16_t* J, int n_, int 
 // product is unconditional; the first word's products start from a zero C
 // operand, so the accumulators need no clearing.  J lists S's positions in
 // ascending order (n > 0 entries); every lane of the wave must call this.
-// This form keeps one block pair at a time (the 512-wide roots, where
-// registers decide occupancy); on rows of 512 positions it replaces sixteen
-// bit tests per lane over up to eight words (tri_mfma1_bits) with one more
-// product per word that holds a member of the block (P14 on R-MAT 22, roots
-// [0, 2^20): 512-wide tier 21.5 -> 14.3 ms).
+// This form keeps one block pair at a time (the 512- and 1024-wide roots,
+// where registers decide occupancy); on rows of 512 positions it replaces
+// sixteen bit tests per lane over up to eight words (tri_mfma1_bits) with one
+// more product per word that holds a member of the block (P14 on R-MAT 22,
+// roots [0, 2^20): 512-wide tier 21.5 -> 14.3 ms), and on the 1024-wide
+// count its registers allow 12 waves per root where the 2x2 tiles of
+// tri_mfma held 8 (23.0 -> 20.7 ms).
 template <class RowT>
 __device__ uint64_t tri_mfma1(const RowT& R, const int16_t* J, int n_, int i_, int whi_, int zr_) {
   const int n = uni(n_), i = uni(i_), whi = uni(whi_), zr = uni(zr_);
@@ -1041,9 +1043,10 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         }
       } else {
         if constexpr (MMA5) {
-          // tiles of block pairs pay for their registers only on the widest
-          // roots; the edge mask is a product from 512 wide up
-          if constexpr (WMAX >= 16) acc += tri_mfma(R, J, n, i, whi, d - 1);
+          // tiles of block pairs pay for their registers only on the 2048-wide
+          // roots (rows in global memory); the edge mask is a product from
+          // 512 wide up
+          if constexpr (WMAX >= 32) acc += tri_mfma(R, J, n, i, whi, d - 1);
           else if constexpr (WMAX >= 8) acc += tri_mfma1(R, J, n, i, whi, d - 1);
           else acc += tri_mfma1_bits(R, J, n, i, whi, d - 1);
         } else if constexpr (K == 5 && WMAX <= 16) {

@@ -134,10 +134,9 @@ __global__ void __launch_bounds__(64 * 16) clique_build_kernel(CliqueArgs a) {
 
 // Waves per root of the count kernel: 12 (three per SIMD, all the LDS
 // allows beside the rows) where the count fits the registers of three waves;
-// the 5- and 7-clique counts hold 2x2 tiles of matrix-core accumulators and
-// would spill there, so they keep 8.
+// the 7-clique count would spill there, so it keeps 8.
 template <int K>
-constexpr int split_count_waves() { return (K == 5 || K == 7) ? 8 : 12; }
+constexpr int split_count_waves() { return (K == 7) ? 8 : 12; }
 
 template <int K>
 __global__ void __launch_bounds__(64 * split_count_waves<K>()) clique_count_kernel(CliqueArgs a) {
