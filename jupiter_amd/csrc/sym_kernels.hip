@@ -95,15 +95,17 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
 // term.  So a root needs one pass over its wedges (v0, v, x) with v, x < v0,
 // not an intersection per pair.  One wave per root; the wedges of 64
 // neighbours v at a time form one flattened stream (consecutive lanes read
-// consecutive elements).  The c_x live
-//  * in the wave's LDS hash table when the root has at most P1_HS / 2
-//    wedges: open addressing on 64-bit slots (key x in the high half, count in
-//    the low half; a slot is claimed by compare-and-swap and counted by a
-//    64-bit add), summed and emptied in one sweep of the table;
-//  * otherwise in the wave's dense global counter array: a first pass adds 1
-//    to c_x, a second takes every counter back to zero with atomicExch and
-//    adds C(c, 2) for the value it took (a repeated x takes 0 after its first
-//    exchange).
+// consecutive elements).  The root's wedge count W picks where the c_x live:
+//  * W <= P1_HS / 2: the wave's LDS hash table, its first 2^t slots with
+//    2^t >= 2W: open addressing on 64-bit slots (key x in the high half,
+//    count in the low half; a slot is claimed by compare-and-swap and counted
+//    by a 64-bit add), summed and emptied in one sweep of those 2^t slots;
+//  * larger W up to a bound: a hash table of the same form over the first
+//    words of the wave's global counter region (keys x + 1, so that zero is
+//    the empty slot), claimed and counted by atomics, emptied by atomicExch;
+//  * otherwise the region as dense counters: a first pass adds 1 to c_x, a
+//    second takes every counter back to zero with atomicExch and adds C(c, 2)
+//    for the value it took (a repeated x takes 0 after its first exchange).
 constexpr uint64_t P1_EMPTY = 0xffffffff00000000ull;  // key -1, count 0
 
 // Calls f(x) for every wedge endpoint x of root v0 (its m0 neighbours below
