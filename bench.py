@@ -333,9 +333,9 @@ def serve_requests(scales, n_req, rate, seed):
     return reqs, np.cumsum(rng.exponential(1.0 / rate, n_req)), pats
 
 
-def run_served(ctx, Gs, reqs, arrivals):
+def run_served(ctx, Gs, reqs, arrivals, lanes=0):
     res = [None] * len(reqs)
-    with J.Server(ctx) as srv:
+    with J.Server(ctx, heavy_lanes=lanes) as srv:
         tickets = [None] * len(reqs)
         start = time.perf_counter()
 

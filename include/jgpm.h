@@ -253,6 +253,9 @@ typedef struct {
   int64_t hbm_budget_bytes;  /* admission budget for per-query intermediates   */
   int32_t max_inflight;      /* queries admitted to the device at once         */
   int32_t slice_roots;       /* roots per scheduling slice (fairness)          */
+  int32_t heavy_lanes;       /* heavy queries running at once, each on a stream
+                                and scratch of its own (0: 4)                  */
+  int32_t reserved;          /* zero                                           */
 } jgpm_serve_config;
 int  jgpm_serve_start(jgpm_ctx_t ctx, const jgpm_serve_config* cfg, jgpm_serve_t* out);
 /* Enqueue a query; returns a ticket.  Thread-safe. */

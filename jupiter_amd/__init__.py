@@ -69,7 +69,7 @@ class _Query(ctypes.Structure):
 
 class _ServeCfg(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
-                ("slice_roots", ctypes.c_int32)]
+                ("slice_roots", ctypes.c_int32), ("heavy_lanes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def lib():
@@ -495,8 +495,8 @@ class Server:
     (jgpm_serve_*).  submit() returns a ticket; wait()/poll() give
     (count, latency_us)."""
 
-    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=0, slice_roots=0):
-        cfg = _ServeCfg(hbm_budget_bytes, max_inflight, slice_roots)
+    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=0, slice_roots=0, heavy_lanes=0):
+        cfg = _ServeCfg(hbm_budget_bytes, max_inflight, slice_roots, heavy_lanes, 0)
         self._h = ctypes.c_void_p()
         _check(lib().jgpm_serve_start(ctx._h, ctypes.byref(cfg), ctypes.byref(self._h)), "serve_start")
 

@@ -14,9 +14,15 @@ namespace jgpm {
 
 static int grid_for(jgpm_ctx_s* c, int per_cu) { return std::max(1, per_cu) * c->num_cu; }
 
+static thread_local jgpm_ctx_s* t_exec = nullptr;
+
+jgpm_ctx_s* exec_ctx(const jgpm_graph_s* g) { return t_exec ? t_exec : g->ctx; }
+ExecScope::ExecScope(jgpm_ctx_s* c) : prev(t_exec) { t_exec = c; }
+ExecScope::~ExecScope() { t_exec = prev; }
+
 // Lazily built per-graph arrays of the symmetric kernels.
 int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t st) {
-  jgpm_ctx_s* c = g->ctx;
+  jgpm_ctx_s* c = exec_ctx(g);
   const int64_t ne = g->sym.nnz;
   if (!g->d_src) {
     if (hipMalloc(&g->d_src, sizeof(int32_t) * std::max<int64_t>(ne, 1)) != hipSuccess) return JGPM_E_NOMEM;
@@ -71,7 +77,7 @@ int ensure_p1_counters(jgpm_ctx_s* c, size_t need, hipStream_t st) {
 
 int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* qctr, int32_t* cnt, size_t cnt_ints,
               hipStream_t st) {
-  jgpm_ctx_s* c = g->ctx;
+  jgpm_ctx_s* c = exec_ctx(g);
   const int32_t nv = (int32_t)g->sym.nv;
   if (re <= rb) return JGPM_OK;
   const int64_t waves = std::min<int64_t>(p1_waves(c, nv), (int64_t)(cnt_ints / std::max<int32_t>(nv, 1)));
@@ -98,7 +104,7 @@ int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* 
 
 int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStream_t st) {
   if (!g->has_sym) return JGPM_E_GRAPHKIND;
-  jgpm_ctx_s* c = g->ctx;
+  jgpm_ctx_s* c = exec_ctx(g);
   if (p == 1) {
     int rc = ensure_p1_counters(c, p1_counter_ints(c, (int32_t)g->sym.nv), st);
     if (rc) return rc;
@@ -150,8 +156,9 @@ int enqueue_vertex_sum(jgpm_graph_s* g, int mode, int64_t rb, int64_t re, int sl
   int rc = ensure_sym_aux(g, mode == 1, false, st);
   if (rc) return rc;
   if (re <= rb) return JGPM_OK;
-  return jk::launch_vertex_sums(g->sym.rp, g->d_tri, rb, re, mode, (unsigned long long*)(g->ctx->d_out + slot),
-                                grid_for(g->ctx, 4), st) == hipSuccess
+  jgpm_ctx_s* c = exec_ctx(g);
+  return jk::launch_vertex_sums(g->sym.rp, g->d_tri, rb, re, mode, (unsigned long long*)(c->d_out + slot),
+                                grid_for(c, 4), st) == hipSuccess
              ? JGPM_OK
              : JGPM_E_HIP;
 }
@@ -200,6 +207,23 @@ int enqueue_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re, int s,
       return enqueue_clique(g, 4, false, rb, re, s + 3, st);
   }
   return JGPM_E_PATTERN;
+}
+
+int prepare_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re) {
+  jgpm_ctx_s* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->dev) != hipSuccess) return JGPM_E_HIP;
+  ExecScope own(c);
+  int rc = JGPM_OK;
+  if (clique_k(pattern)) {
+    if (g->has_dag) rc = ensure_huge_ids(g);
+  } else if (pattern == JGPM_P1) {
+    if (g->has_sym && rb == 0 && re == g->sym.nv) rc = ensure_rank_csr(g, c->st);
+  } else if (pattern >= JGPM_P2 && pattern <= JGPM_P12) {
+    if (g->has_sym) rc = ensure_sym_aux(g, pattern_uses_tri(pattern), pattern == JGPM_P3, c->st);
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess && !rc) rc = JGPM_E_HIP;
+  return rc;
 }
 
 uint64_t finish_pattern(int pattern, const uint64_t* h) {

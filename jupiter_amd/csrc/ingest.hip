@@ -418,7 +418,7 @@ namespace jgpm {
 // edge, so each new list is ascending; the row of ord[v] has d(v) entries.
 int ensure_rank_csr(jgpm_graph_s* g, hipStream_t st) {
   if (g->rsym.ci || !g->has_sym) return JGPM_OK;
-  jgpm_ctx_s* c = g->ctx;
+  jgpm_ctx_s* c = exec_ctx(g);
   if (c->capturing) return JGPM_E_STATE;
   int rc = ensure_sym_aux(g, false, false, st);  // COO sources
   if (rc) return rc;

@@ -442,10 +442,19 @@ static const char* const kTierName[jk::NUM_TIERS] = {"clique_wave64", "clique_ld
 
 int glb_tier_grid(const jgpm_ctx_s* c) { return 4 * c->num_cu; }
 
+int ensure_huge_ids(jgpm_graph_s* g) {
+  if (g->huge_d || g->huge_h.empty()) return JGPM_OK;
+  if (exec_ctx(g)->capturing) return JGPM_E_STATE;
+  if (hipMalloc(&g->huge_d, g->huge_h.size() * sizeof(int32_t)) != hipSuccess) return JGPM_E_NOMEM;
+  HIP_TRY(hipMemcpy(g->huge_d, g->huge_h.data(), g->huge_h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  g->bytes += (int64_t)(g->huge_h.size() * sizeof(int32_t));
+  return JGPM_OK;
+}
+
 int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, int out_slot, hipStream_t st,
                    bool split_slots, int first_tier, uint64_t* out_ptr, unsigned* qctr_base, uint64_t* gscr,
                    size_t gscr_words) {
-  jgpm_ctx_s* c = g->ctx;
+  jgpm_ctx_s* c = exec_ctx(g);
   if (!g->huge_h.empty()) {
     // roots wider than 2048 (the DAG upload keeps them in huge_h): the
     // level-nest kernel, its lists in the row scratch
@@ -467,12 +476,8 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
         if (s2) return s2;
         scr = reinterpret_cast<int32_t*>(c->d_scr);
       }
-      if (!g->huge_d) {
-        if (c->capturing) return JGPM_E_STATE;
-        if (hipMalloc(&g->huge_d, g->huge_h.size() * sizeof(int32_t)) != hipSuccess) return JGPM_E_NOMEM;
-        HIP_TRY(hipMemcpy(g->huge_d, g->huge_h.data(), g->huge_h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        g->bytes += (int64_t)(g->huge_h.size() * sizeof(int32_t));
-      }
+      int s3 = ensure_huge_ids(g);
+      if (s3) return s3;
       jk::HugeArgs h{};
       h.rp = g->dag.rp;
       h.ci = g->dag.ci;

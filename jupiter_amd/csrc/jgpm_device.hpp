@@ -78,6 +78,29 @@ struct jgpm_graph_s {
 };
 
 namespace jgpm {
+// The context whose stream, accumulators, counters and scratch the dispatch
+// routines use for graph g: g's own context, unless an ExecScope on the
+// calling thread names another context of the same device (the serving
+// loop's heavy-request lanes run graphs resident on the server's context on
+// lane contexts of their own, so their launches overlap on the device).
+jgpm_ctx_s* exec_ctx(const jgpm_graph_s* g);
+struct ExecScope {
+  jgpm_ctx_s* prev;
+  explicit ExecScope(jgpm_ctx_s* c);
+  ~ExecScope();
+  ExecScope(const ExecScope&) = delete;
+  ExecScope& operator=(const ExecScope&) = delete;
+};
+// Builds every per-graph array that `pattern` over roots [rb, re) would
+// otherwise build on first use (COO sources, per-edge triangles, neighbour
+// degree sums, the degree-order relabelling, the wide-root ids), on g's own
+// context under its lock, and waits for them: after it, a count of that
+// query allocates and writes nothing of g's, so it may run on another
+// context's stream while other work on g proceeds.
+int prepare_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re);
+// Device copy of the ids of the roots wider than the widest row tier
+// (g->huge_d), made on first use.
+int ensure_huge_ids(jgpm_graph_s* g);
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
 // Address of this device's static scratch array of jgpm_count_noalloc
 // (noalloc.hip), taken once per device with `dev` current.

@@ -16,10 +16,16 @@
 //    with the tails of earlier ones on the device;
 //  * heavy requests: a request whose work (edges of its range times the
 //    pattern weight) exceeds one slice would fill the device by itself, so
-//    it is not sliced: it runs whole with the single-query kernels, between
-//    the batched launches of the light requests (its count is then exactly
-//    jgpm_count's, from the same kernels, including the degree-order
-//    relabelling a whole-graph 4-cycle count uses);
+//    it is not sliced: it runs whole with the single-query kernels (its count
+//    is then exactly jgpm_count's, from the same kernels, including the
+//    degree-order relabelling a whole-graph 4-cycle count uses).  Heavy
+//    requests run on lanes: each lane is a thread with a context of its own
+//    (stream, accumulators, counters, scratch) on the same device, so up to
+//    heavy_lanes heavy requests and the light batches overlap on the device,
+//    filling the tail of one query's launches with the next query's work.
+//    Before a heavy request goes to a lane, the scheduler builds every
+//    per-graph array its kernels would create on first use (on the graph's
+//    own context, waited for), so a lane only reads the shared graph;
 //  * completion: a request finishes when its last slice is counted; its
 //    latency is submit -> result visible on the host.
 // Counts are sums over disjoint root slices, so a result is bit-identical to
@@ -49,8 +55,24 @@ struct Req {
   int64_t rb, re, cur;
   uint64_t count = 0;
   int64_t footprint = 0;
-  bool heavy = false;  // runs whole, with the single-query kernels (not sliced into batches)
+  bool heavy = false;       // runs whole, with the single-query kernels (not sliced into batches)
+  bool dispatched = false;  // a heavy request handed to a lane
+  int status = JGPM_OK;
   clk::time_point t_submit;
+};
+
+struct HeavyDone {
+  uint64_t ticket;
+  uint64_t count;
+  int status;
+};
+
+// One heavy request at a time, on a context of its own.
+struct Lane {
+  jgpm_ctx_s* ctx = nullptr;
+  std::thread th;
+  bool busy = false;  // guarded by the server's mutex, as is job
+  Req job;
 };
 
 struct Done {
@@ -89,6 +111,10 @@ struct jgpm_serve_s {
   int fatal = JGPM_OK;
   std::thread th;
   int64_t slice_edges = 1 << 22;
+  std::vector<Lane*> lanes;
+  std::condition_variable cv_lane;
+  std::vector<HeavyDone> heavy_done;  // finished on a lane, not yet merged by the scheduler
+  bool lanes_stop = false;
 };
 
 static bool is_dag_pattern(int p) { return jgpm::clique_k(p) != 0 && p != JGPM_I_K4; }
@@ -108,15 +134,77 @@ static int64_t footprint(jgpm_graph_s* g, int p, const jgpm_ctx_s* c, int grid) 
   return b;
 }
 
+// Runs a heavy request whole on lane context lc; the graph's lazy arrays are
+// built (prepare_pattern), so the enqueue only reads the graph, under its
+// context's lock like every other reader.
+static int run_heavy(jgpm_ctx_s* main, jgpm_ctx_s* lc, const Req& r, uint64_t* count) {
+  std::lock_guard<std::mutex> own(lc->mu);
+  int st = hipSetDevice(lc->dev) == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+  {
+    jgpm::ExecScope scope(lc);
+    std::lock_guard<std::mutex> gl(main->mu);
+    if (!st) st = jgpm::ctx_reset_counters(lc, lc->st);
+    if (!st) {
+      lc->nprof = 0;
+      st = jgpm::enqueue_pattern(r.g, r.pattern, r.rb, r.re, 0, lc->st);
+    }
+  }
+  uint64_t hc[CTX_SLOTS];
+  if (!st && hipMemcpyAsync(hc, lc->d_out, sizeof(hc), hipMemcpyDeviceToHost, lc->st) != hipSuccess) st = JGPM_E_HIP;
+  if (hipStreamSynchronize(lc->st) != hipSuccess && !st) st = JGPM_E_HIP;
+  if (!st) *count = jgpm::finish_pattern(r.pattern, hc);
+  return st;
+}
+
+static void lane_loop(jgpm_serve_s* s, Lane* L) {
+  for (;;) {
+    Req r;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_lane.wait(lk, [&] { return s->lanes_stop || L->busy; });
+      if (!L->busy) return;
+      r = L->job;
+    }
+    uint64_t count = 0;
+    const int st = run_heavy(s->ctx, L->ctx, r, &count);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      L->busy = false;
+      s->heavy_done.push_back(HeavyDone{r.ticket, count, st});
+    }
+    s->cv_work.notify_all();
+  }
+}
+
+static Lane* free_lane(jgpm_serve_s* s) {
+  for (Lane* L : s->lanes)
+    if (!L->busy) return L;
+  return nullptr;
+}
+
 static void serve_loop(jgpm_serve_s* s) {
   jgpm_ctx_s* c = s->ctx;
   std::vector<jgpm_query> qs;
   std::vector<size_t> owner;
   std::vector<uint64_t> h;
   for (;;) {
+    std::vector<size_t> to_lane;  // heavy requests to hand to lanes, oldest first
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->active.empty(); });
+      // work: an admissible arrival, lane results, light slices, or a heavy
+      // request and a free lane
+      auto has_work = [&] {
+        if (!s->heavy_done.empty()) return true;
+        if (!s->pending.empty() && (int)s->active.size() < s->cfg.max_inflight &&
+            (s->active.empty() || s->inflight_bytes + s->pending.front().footprint <= s->cfg.hbm_budget_bytes))
+          return true;
+        for (const Req& r : s->active) {
+          if (r.cur >= r.re) continue;
+          if (!r.heavy || (!r.dispatched && free_lane(s))) return true;
+        }
+        return false;
+      };
+      s->cv_work.wait(lk, [&] { return has_work() || (s->stop && s->active.empty() && s->pending.empty()); });
       if (s->stop && s->active.empty() && s->pending.empty()) return;
       // admission in arrival order
       while (!s->pending.empty() && (int)s->active.size() < s->cfg.max_inflight) {
@@ -126,18 +214,44 @@ static void serve_loop(jgpm_serve_s* s) {
         s->active.push_back(r);
         s->pending.pop_front();
       }
+      // results of the lanes
+      for (const HeavyDone& d : s->heavy_done)
+        for (Req& r : s->active)
+          if (r.ticket == d.ticket) {
+            r.count = d.count;
+            r.status = d.status;
+            r.cur = r.re;
+          }
+      s->heavy_done.clear();
+      int free = 0;
+      for (Lane* L : s->lanes) free += !L->busy;
+      for (size_t i = 0; i < s->active.size() && (int)to_lane.size() < free; ++i) {
+        const Req& r = s->active[i];
+        if (r.heavy && !r.dispatched && r.cur < r.re) to_lane.push_back(i);
+      }
     }
-    // next slice of every light active request (the active list is only touched here)
-    qs.clear();
-    owner.clear();
-    int heavy = -1;
-    for (size_t i = 0; i < s->active.size(); ++i) {
+    // heavy requests to free lanes (the active list is only touched by this thread)
+    for (size_t i : to_lane) {
       Req& r = s->active[i];
-      if (r.cur >= r.re) continue;
-      if (r.heavy) {
-        if (heavy < 0) heavy = (int)i;
+      const int rc = jgpm::prepare_pattern(r.g, r.pattern, r.rb, r.re);
+      std::lock_guard<std::mutex> lk(s->mu);
+      r.dispatched = true;
+      if (rc) {
+        r.status = rc;
+        r.cur = r.re;
         continue;
       }
+      Lane* L = free_lane(s);  // lanes only turn free meanwhile, so one is
+      L->job = r;
+      L->busy = true;
+      s->cv_lane.notify_all();
+    }
+    // next slice of every light active request
+    qs.clear();
+    owner.clear();
+    for (size_t i = 0; i < s->active.size(); ++i) {
+      Req& r = s->active[i];
+      if (r.cur >= r.re || r.heavy) continue;
       jgpm_graph_s* g = r.g;
       const std::vector<int64_t>& rp = is_dag_pattern(r.pattern) ? g->dag_rp_h : g->sym_rp_h;
       const int64_t budget = std::max<int64_t>(1, (int64_t)(s->slice_edges / pattern_weight(r.pattern)));
@@ -151,49 +265,35 @@ static void serve_loop(jgpm_serve_s* s) {
     }
     int st = JGPM_OK;
     if (!qs.empty()) {
-      std::lock_guard<std::mutex> lk(c->mu);
-      st = (hipSetDevice(c->dev) == hipSuccess) ? JGPM_OK : JGPM_E_HIP;
-      if (!st) st = jgpm::ctx_reset_counters(c, c->st);
-      if (!st) {
-        c->nprof = 0;
-        st = jgpm::enqueue_batch_q(c, qs.data(), (int)qs.size(), c->st);
+      {
+        // the context lock covers the enqueue; later work on the stream runs after these launches
+        std::lock_guard<std::mutex> lk(c->mu);
+        st = (hipSetDevice(c->dev) == hipSuccess) ? JGPM_OK : JGPM_E_HIP;
+        if (!st) st = jgpm::ctx_reset_counters(c, c->st);
+        if (!st) {
+          c->nprof = 0;
+          st = jgpm::enqueue_batch_q(c, qs.data(), (int)qs.size(), c->st);
+        }
+        h.assign(qs.size(), 0);
+        if (!st && hipMemcpyAsync(h.data(), c->d_bcnt, sizeof(uint64_t) * qs.size(), hipMemcpyDeviceToHost, c->st) !=
+                       hipSuccess)
+          st = JGPM_E_HIP;
       }
-      h.assign(qs.size(), 0);
-      if (!st && hipMemcpyAsync(h.data(), c->d_bcnt, sizeof(uint64_t) * qs.size(), hipMemcpyDeviceToHost, c->st) !=
-                     hipSuccess)
-        st = JGPM_E_HIP;
       if (hipStreamSynchronize(c->st) != hipSuccess && !st) st = JGPM_E_HIP;
-    }
-    // then the oldest heavy request, whole, with the single-query kernels
-    uint64_t heavy_count = 0;
-    if (!st && heavy >= 0) {
-      Req& r = s->active[heavy];
-      std::lock_guard<std::mutex> lk(c->mu);
-      st = (hipSetDevice(c->dev) == hipSuccess) ? JGPM_OK : JGPM_E_HIP;
-      if (!st) st = jgpm::ctx_reset_counters(c, c->st);
-      if (!st) {
-        c->nprof = 0;
-        st = jgpm::enqueue_pattern(r.g, r.pattern, r.rb, r.re, 0, c->st);
-      }
-      uint64_t hc[CTX_SLOTS];
-      if (!st && hipMemcpyAsync(hc, c->d_out, sizeof(hc), hipMemcpyDeviceToHost, c->st) != hipSuccess) st = JGPM_E_HIP;
-      if (hipStreamSynchronize(c->st) != hipSuccess && !st) st = JGPM_E_HIP;
-      if (!st) heavy_count = jgpm::finish_pattern(r.pattern, hc);
     }
     const auto now = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
     for (size_t k = 0; k < qs.size(); ++k) s->active[owner[k]].count += h[k];
-    if (!st && heavy >= 0) {
-      s->active[heavy].count = heavy_count;
-      s->active[heavy].cur = s->active[heavy].re;
-    }
-    // finished requests (or all, after an error) leave the device
+    // finished requests (all light ones after a failed batch) leave the device;
+    // a heavy request on a lane finishes when its result is merged
     std::vector<Req> keep;
     for (Req& r : s->active) {
-      if (st || r.cur >= r.re) {
+      const bool failed = st && !r.heavy;
+      if (failed || r.cur >= r.re) {
         double us = std::chrono::duration<double, std::micro>(now - r.t_submit).count();
-        s->done[r.ticket] = Done{r.count, us, st};
+        s->done[r.ticket] = Done{r.count, us, failed ? st : r.status};
         s->inflight_bytes -= r.footprint;
+        if (r.status == JGPM_E_HIP && !s->fatal) s->fatal = r.status;
       } else {
         keep.push_back(r);
       }
@@ -204,6 +304,21 @@ static void serve_loop(jgpm_serve_s* s) {
   }
 }
 
+// Ends the lane threads (idle once the scheduler has drained) and their contexts.
+static void stop_lanes(jgpm_serve_s* s) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lanes_stop = true;
+  }
+  s->cv_lane.notify_all();
+  for (Lane* L : s->lanes) {
+    if (L->th.joinable()) L->th.join();
+    if (L->ctx) (void)jgpm_shutdown(L->ctx);
+    delete L;
+  }
+  s->lanes.clear();
+}
+
 extern "C" int jgpm_serve_start(jgpm_ctx_t ctx, const jgpm_serve_config* cfg, jgpm_serve_t* out) {
   if (!ctx || !out) return JGPM_E_INVALID;
   auto* s = new (std::nothrow) jgpm_serve_s();
@@ -212,6 +327,21 @@ extern "C" int jgpm_serve_start(jgpm_ctx_t ctx, const jgpm_serve_config* cfg, jg
   if (cfg) s->cfg = *cfg;
   if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = ctx->hbm / 2;
   if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 256;
+  if (s->cfg.heavy_lanes <= 0) s->cfg.heavy_lanes = 4;
+  s->cfg.heavy_lanes = std::min(s->cfg.heavy_lanes, 16);
+  for (int i = 0; i < s->cfg.heavy_lanes; ++i) {
+    auto* L = new (std::nothrow) Lane();
+    int st = L ? jgpm_init(ctx->dev, &L->ctx) : JGPM_E_NOMEM;
+    if (st) {
+      delete L;
+      stop_lanes(s);
+      delete s;
+      return st;
+    }
+    L->ctx->tune = ctx->tune;
+    s->lanes.push_back(L);
+    L->th = std::thread(lane_loop, s, L);
+  }
   s->th = std::thread(serve_loop, s);
   *out = s;
   return JGPM_OK;
@@ -303,6 +433,7 @@ extern "C" int jgpm_serve_stop(jgpm_serve_t s) {
   }
   s->cv_work.notify_all();
   if (s->th.joinable()) s->th.join();
+  stop_lanes(s);
   delete s;
   return JGPM_OK;
 }

@@ -82,3 +82,32 @@ def test_budget_and_pattern_rejection(ctx):
             srv.poll(t)
         assert e.value.status == -1
     G.free()
+
+
+def test_heavy_lanes_on_fresh_graphs(ctx):
+    """Heavy requests run on lanes, each a context of its own, so several are
+    on the device at once.  On graphs whose per-graph arrays are not built yet
+    (COO sources, edge triangles, neighbour-degree sums, the degree-order
+    relabelling of a whole-graph 4-cycle count, the ids of a root wider than
+    2048) the scheduler builds them before a lane runs the query; every count
+    equals the single-query count computed afterwards, with four lanes and
+    with one."""
+    from gen import hub_graph
+    gs = [ctx.upload(J.rmat(15, 16, seed=s)) for s in (7, 8)]
+    hub = ctx.upload(hub_graph(2600, 0.006, seed=26))
+    reqs = [(gs[0], "P1", None), (gs[1], "P3", None), (gs[0], "P7", None), (gs[1], "P13", None),
+            (gs[0], "P14", None), (gs[1], "P1", None), (gs[0], "P3", None), (hub, "TC", None),
+            (hub, "P13", None), (gs[1], "P4", (30000, 32768)), (gs[0], "P2", None), (gs[1], "P1", (50, 20000))]
+    got = {}
+    try:
+        for lanes in (4, 1):
+            print(f"[case] serving {len(reqs)} heavy requests, {lanes} lane(s)", flush=True)
+            with J.Server(ctx, heavy_lanes=lanes) as srv:
+                tickets = [srv.submit(G, p, r) for G, p, r in reqs]
+                got[lanes] = [srv.wait(t)[0] for t in tickets]
+        alone = [G.count(p, r)[0] for G, p, r in reqs]
+        assert got[4] == alone
+        assert got[1] == alone
+    finally:
+        for G in gs + [hub]:
+            G.free()
