@@ -989,6 +989,16 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
         return __shfl(t, 0);
       }
     };
+    // the lane's share of this root's 5-cliques on the 256- and 512-wide
+    // tiers, in 32 bits: at most 512 rows of at most 136 block pairs, each
+    // adding at most 16 entries of at most 512, below 2^30.  It is kept in
+    // the wave's successor-bitmap slot (unused by the matrix-core count) and
+    // added to without a return value: a sum held in registers across the
+    // row loop costs those tiers a register spill and reload per row.
+    uint32_t* racc = reinterpret_cast<uint32_t*>(Rc) + ln;
+    if constexpr (MMA5 && WMAX <= 8) *racc = 0u;
+    static_assert(!MMA5 || WMAX > 8 || (long)(64 * WMAX) * (WMAX * (2 * WMAX + 1)) * 16 * (64 * WMAX) < (1l << 32),
+                  "the 32-bit root share of the narrow 5-clique tiers could overflow");
     for (int i = DYN ? next_row(0) : wv; i < d; i = next_row(i)) {
       // population and nonzero word window of row i (W <= 32 words)
       const uint64_t myword = ln < W ? R(i, ln) : 0ull;
@@ -1001,8 +1011,10 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       for (int w = wlo; w <= whi; ++w) {
         const uint64_t word = R(i, w);
         if ((word >> ln) & 1ull) {
-          uint64_t below = ln ? (word & ((~0ull) >> (64 - ln))) : 0ull;
-          J[n + __popcll(below)] = (int16_t)((w << 6) | ln);
+          // set bits of the word below this lane's bit
+          const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0u));
+          J[n + below] = (int16_t)((w << 6) | ln);
         }
         n += __popcll(word);
       }
@@ -1047,8 +1059,13 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
           // roots (rows in global memory); the edge mask is a product from
           // 512 wide up
           if constexpr (WMAX >= 32) acc += tri_mfma(R, J, n, i, whi, d - 1);
-          else if constexpr (WMAX >= 8) acc += tri_mfma1(R, J, n, i, whi, d - 1);
-          else acc += tri_mfma1_bits(R, J, n, i, whi, d - 1);
+          else if constexpr (WMAX >= 16) acc += tri_mfma1(R, J, n, i, whi, d - 1);
+          else if constexpr (WMAX >= 8)
+            __hip_atomic_fetch_add(racc, (uint32_t)tri_mfma1(R, J, n, i, whi, d - 1), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+          else
+            __hip_atomic_fetch_add(racc, (uint32_t)tri_mfma1_bits(R, J, n, i, whi, d - 1), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
         } else if constexpr (K == 5 && WMAX <= 16) {
           // 5-cliques: triangles inside S.  One flat loop per lane over a
           // work list shared through an LDS counter: a step either takes the
@@ -1126,6 +1143,7 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       }
       __syncwarp();
     }
+    if constexpr (MMA5 && WMAX <= 8) acc += __hip_atomic_load(racc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   }
   Grp<G>::sync();
