@@ -984,9 +984,14 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       if constexpr (!DYN) {
         return cur + G;
       } else {
+        // the counter's LDS address from a zero made here (the empty asm
+        // keeps it from being hoisted): a loop-invariant address register
+        // would be spilled across the row's matrix-core work and reloaded
+        int z = 0;
+        asm volatile("" : "+v"(z));
         int t = 0;
-        if (ln == 0) t = atomicAdd(&s_wtot[G], 1);
-        return __shfl(t, 0);
+        if (ln == 0) t = atomicAdd(s_wtot + G + z, 1);
+        return __builtin_amdgcn_readfirstlane(t);
       }
     };
     // the lane's share of this root's 5-cliques on the 256- and 512-wide
@@ -999,7 +1004,11 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
     if constexpr (MMA5 && WMAX <= 8) *racc = 0u;
     static_assert(!MMA5 || WMAX > 8 || (long)(64 * WMAX) * (WMAX * (2 * WMAX + 1)) * 16 * (64 * WMAX) < (1l << 32),
                   "the 32-bit root share of the narrow 5-clique tiers could overflow");
-    for (int i = DYN ? next_row(0) : wv; i < d; i = next_row(i)) {
+    // the next row is taken before this row's work, so its counter round
+    // trip overlaps the work
+    int inext = 0;
+    for (int i = DYN ? next_row(0) : wv; i < d; i = inext) {
+      inext = next_row(i);
       // population and nonzero word window of row i (W <= 32 words)
       const uint64_t myword = ln < W ? R(i, ln) : 0ull;
       const uint64_t nz = __ballot(myword != 0ull);
