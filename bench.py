@@ -184,7 +184,7 @@ def cpu_baseline(dag, roots_end, seed=7, per_stratum=(256, 64, 32, 16, 8), batch
             "sample": f"stratified by clique tier: {nroots} of {roots_end} roots "
                       f"({', '.join(str(len(t)) + '/' + str(n) for n, t in strata)}), P13+P14, one thread "
                       f"{secs1:.1f}s; all {threads} threads: speedup {speedup:.1f} measured on the "
-                      f"{len(light)} light-stratum roots ({light1:.1f}s -> {wall:.1f}s)",
+                      f"{len(light)} light-stratum roots ({light1:.2f}s -> {wall:.3f}s)",
             "est_s_per_step": round(est1 / speedup, 1), "std_err_s": round(se1 / speedup, 1),
             "parallel_speedup": round(speedup, 2),
             "one_thread": {"value": round(q / est1, 7), "est_s_per_step": round(est1, 1), "std_err_s": round(se1, 1)}}
