@@ -633,8 +633,10 @@ __device__ uint64_t tri4_mfma(const RowT& R, int d_, int W_, int wv_, int G) {
   return s;
 }
 
-// The same count with 2 x 2 tiles of block pairs (widest roots): each word's
-// four operand slices feed four independent count accumulators.  The edge
+// The same count with 2 x 2 tiles of block pairs (the 2048-wide roots, whose
+// rows are read from global memory, so each loaded operand feeding two
+// products pays): each word's four operand slices feed four independent
+// count accumulators.  The edge
 // mask is the product E_Y R_X^T of one-hot rows at the Y members' own
 // positions, accumulated on the words that hold them (q0 .. qm).  Pairs of
 // the tile outside the triangle X <= Y, or with absent blocks, have an empty
