@@ -93,6 +93,9 @@ int launch_p1(jgpm_graph_s* g, int64_t rb, int64_t re, uint64_t* out, unsigned* 
   a.rb = rb;
   a.re = re;
   a.nv = nv;
+  // on the relabelling the hubs rank last: deal them first, so the widest
+  // roots do not run alone at the end of the launch
+  a.desc = ranked ? 1 : 0;
   a.qctr = qctr;
   a.cnt = cnt;
   a.out = out;

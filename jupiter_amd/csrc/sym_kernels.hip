@@ -268,8 +268,9 @@ __global__ void __launch_bounds__(64 * P1_WPB) p1_wedge_kernel(P1Args a) {
   for (;;) {
     int64_t t = 0;
     if (ln == 0) t = (int64_t)atomicAdd(a.qctr, 1u);
-    const int64_t r = a.rb + __shfl(t, 0);
-    if (r >= a.re) break;
+    t = __shfl(t, 0);
+    if (a.rb + t >= a.re) break;
+    const int64_t r = a.desc ? a.re - 1 - t : a.rb + t;
     acc += p1_root<LOG>(a.rp, a.ci, (int32_t)r, tab, s_off[wv], s_lb[wv], cnt, a.nv);
   }
   if (ln == 0 && acc) atomicAdd((unsigned long long*)a.out, (unsigned long long)acc);
@@ -302,8 +303,9 @@ __global__ void __launch_bounds__(64 * P1_WPB) p1_batch_kernel(P1BatchArgs a) {
       if (a.p1cs[mid] <= t) lo = mid; else hi = mid;
     }
     const BQ& Q = a.q[lo];
-    const uint64_t c =
-        p1_root<LOG>(Q.rp, Q.ci, (int32_t)(Q.base + (t - a.p1cs[lo])), tab, s_off[wv], s_lb[wv], cnt, a.cnt_stride);
+    const int64_t k = t - a.p1cs[lo];
+    const int64_t root = Q.desc ? Q.base + (a.p1cs[lo + 1] - a.p1cs[lo]) - 1 - k : Q.base + k;
+    const uint64_t c = p1_root<LOG>(Q.rp, Q.ci, (int32_t)root, tab, s_off[wv], s_lb[wv], cnt, a.cnt_stride);
     if (ln == 0 && c) atomicAdd((unsigned long long*)(a.counts + lo), (unsigned long long)c);
   }
 }
