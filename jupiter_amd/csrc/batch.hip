@@ -228,7 +228,6 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
       b.base = rb;
       b.ntasks = 0;
       b.chunk = 1;
-      b.desc = whole && g->rsym.ci ? 1 : 0;  // hubs rank last: dealt first
     } else {
       const int p = kind - 100;
       int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);

@@ -44,8 +44,6 @@ struct BQ {
   int64_t ntasks;  // roots or edges
   int32_t kind;    // 3..7: clique size; 100 + P: symmetric pattern P
   int32_t chunk;   // roots / edges per chunk
-  int32_t desc;    // 4-cycles (kind 101): roots dealt from the last down
-  int32_t pad;
 };
 
 hipError_t launch_clique(int k, bool bytes, int tier, const CliqueArgs& a, int grid, hipStream_t st);
