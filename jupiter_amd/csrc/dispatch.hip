@@ -32,7 +32,11 @@ int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t s
   if (need_tri && !g->d_tri) {
     if (hipMalloc(&g->d_tri, sizeof(int32_t) * std::max<int64_t>(ne, 1)) != hipSuccess) return JGPM_E_NOMEM;
     g->bytes += 4 * ne;
+    // its own queue counter, zeroed here: the builder runs from single
+    // queries, batch plans and the serving scheduler alike, and not every
+    // caller resets the context's counters first
     unsigned long long* q = c->d_q64 + (CTX_SLOTS - 1);
+    if (zero_u64(reinterpret_cast<uint64_t*>(q), 1, st)) return JGPM_E_HIP;
     prof_begin(c, st, "edge_triangles");
     hipError_t e = jk::launch_aux_tri(g->sym.rp, g->sym.ci, g->d_src, ne, g->d_tri, q, grid_for(c, 4), st);
     prof_end(c, st);
@@ -114,6 +118,20 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
     if (rb == 0 && re == g->sym.nv && (rc = ensure_rank_csr(g, st))) return rc;
     return launch_p1(g, rb, re, c->d_out + slot, c->d_q + slot, c->d_p1cnt, c->p1cnt_ints, st);
   }
+  if (p == 2) {
+    // diamonds from the per-edge triangle array: the array's one pass of
+    // intersections (k_edge_tri) costs about what one P2 pass costs, and
+    // every later diamond or induced-motif query on the graph then only
+    // reads it (a few bytes per edge)
+    int rc = ensure_sym_aux(g, true, false, st);
+    if (rc) return rc;
+    if (re <= rb) return JGPM_OK;
+    prof_begin(c, st, "sym_p2");
+    hipError_t e = jk::launch_vertex_sums(g->sym.rp, g->sym.ci, g->d_tri, rb, re, 2,
+                                          (unsigned long long*)(c->d_out + slot), grid_for(c, 4), st);
+    prof_end(c, st);
+    return e == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+  }
   // The triangle array is shared by later queries; its counter slot is
   // reset per query by ctx_reset_counters before it is used here.
   int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
@@ -160,7 +178,7 @@ int enqueue_vertex_sum(jgpm_graph_s* g, int mode, int64_t rb, int64_t re, int sl
   if (rc) return rc;
   if (re <= rb) return JGPM_OK;
   jgpm_ctx_s* c = exec_ctx(g);
-  return jk::launch_vertex_sums(g->sym.rp, g->d_tri, rb, re, mode, (unsigned long long*)(c->d_out + slot),
+  return jk::launch_vertex_sums(g->sym.rp, g->sym.ci, g->d_tri, rb, re, mode, (unsigned long long*)(c->d_out + slot),
                                 grid_for(c, 4), st) == hipSuccess
              ? JGPM_OK
              : JGPM_E_HIP;
@@ -217,13 +235,15 @@ int prepare_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re) {
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->dev) != hipSuccess) return JGPM_E_HIP;
   ExecScope own(c);
+  c->nprof = 0;
   int rc = JGPM_OK;
   if (clique_k(pattern)) {
     if (g->has_dag) rc = ensure_huge_ids(g);
   } else if (pattern == JGPM_P1) {
     if (g->has_sym && rb == 0 && re == g->sym.nv) rc = ensure_rank_csr(g, c->st);
   } else if (pattern >= JGPM_P2 && pattern <= JGPM_P12) {
-    if (g->has_sym) rc = ensure_sym_aux(g, pattern_uses_tri(pattern), pattern == JGPM_P3, c->st);
+    if (g->has_sym)
+      rc = ensure_sym_aux(g, pattern == JGPM_P2 || pattern_uses_tri(pattern), pattern == JGPM_P3, c->st);
   }
   if (hipStreamSynchronize(c->st) != hipSuccess && !rc) rc = JGPM_E_HIP;
   return rc;

@@ -60,6 +60,11 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
 
 namespace jgpm {
 
+int zero_u64(uint64_t* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n, 0ull);
+  return hipGetLastError() == hipSuccess ? JGPM_OK : JGPM_E_HIP;
+}
+
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(CTX_SLOTS), 0, st, c->d_out, CTX_SLOTS, 0ull);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(CTX_SLOTS), 0, st, c->d_q, CTX_SLOTS, 0u);

@@ -102,6 +102,8 @@ int prepare_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re);
 // (g->huge_d), made on first use.
 int ensure_huge_ids(jgpm_graph_s* g);
 int ctx_reset_counters(jgpm_ctx_s* c, hipStream_t st);
+// p[0..n) = 0 on st, by a fill kernel (no memset node: callers may be captured).
+int zero_u64(uint64_t* p, int n, hipStream_t st);
 // Address of this device's static scratch array of jgpm_count_noalloc
 // (noalloc.hip), taken once per device with `dev` current.
 int noalloc_setup(int dev, uint64_t** out);

@@ -376,10 +376,13 @@ __global__ void k_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int64_t
   }
 }
 
-// Per-vertex pattern sums over the root range: 3-stars C(d,3) and tailed
-// triangles t_v (d_v - 2) with t_v = (sum of tri over v's edges) / 2.
-__global__ void k_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
-                              unsigned long long* out) {
+// Per-vertex pattern sums over the root range: 3-stars C(d,3) (mode 0),
+// tailed triangles t_v (d_v - 2) with t_v = (sum of tri over v's edges) / 2
+// (mode 1), and diamonds (P2, mode 2): the reference's per-edge value
+// C(|N(v0) ∩ N(v1)|, 2) for the edges (v0, v1), v1 < v0, of the roots v0,
+// read from the per-edge triangle array (a lower prefix of each list).
+__global__ void k_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_t* tri, int64_t rb, int64_t re,
+                              int mode, unsigned long long* out) {
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   uint64_t acc = 0;
@@ -387,6 +390,11 @@ __global__ void k_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb,
     const int64_t d = rp[v + 1] - rp[v];
     if (mode == 0) {
       if (lane() == 0 && d >= 3) acc += (uint64_t)(d * (d - 1) / 2) * (uint64_t)(d - 2) / 3;
+    } else if (mode == 2) {
+      for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) {
+        if (ci[e] >= v) break;  // ascending list: the rest are above v
+        acc += (uint64_t)C2(tri[e]);
+      }
     } else {
       int64_t s = 0;
       for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) s += tri[e];
@@ -438,9 +446,9 @@ hipError_t launch_aux_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int
   hipLaunchKernelGGL(k_nsum, dim3(grid), dim3(256), 0, st, rp, ci, nv, nsum);
   return hipGetLastError();
 }
-hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
-                              unsigned long long* out, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_vertex_sums, dim3(grid), dim3(256), 0, st, rp, tri, rb, re, mode, out);
+hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_t* tri, int64_t rb, int64_t re,
+                              int mode, unsigned long long* out, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_vertex_sums, dim3(grid), dim3(256), 0, st, rp, ci, tri, rb, re, mode, out);
   return hipGetLastError();
 }
 

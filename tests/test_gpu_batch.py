@@ -82,3 +82,28 @@ def test_batch_edge_cases(ctx):
     got, _ = ctx.count_batch([(G, "TC", 3, 3), (G, "P2", 0, 0)])
     assert got == [0, 0]
     G.free()
+
+
+def test_batch_builds_edge_triangles_after_another_build(ctx):
+    """A batch whose plan builds a graph's per-edge triangle array right after
+    a single query built another graph's (the builder's queue counter left at
+    that graph's edge count) counts every edge: P3, P7 and P11 in batches on
+    fresh graphs equal the oracle."""
+    a = J.rmat(10, 16, seed=41)
+    b = J.barabasi_albert(1500, 6, seed=42)
+    Ga, Gb = ctx.upload(a), ctx.upload(b)
+    try:
+        _say("batch after a single-query triangle build")
+        assert Ga.count("P3")[0] == O.count(a.rowptr, a.col, J.PATTERNS["P3"])
+        got, _ = ctx.count_batch([(Gb, "P3", 0, None), (Gb, "P7", 0, None)])
+        assert got == [O.count(b.rowptr, b.col, J.PATTERNS["P3"]), O.count(b.rowptr, b.col, J.PATTERNS["P7"])]
+        Gc = ctx.upload(a)
+        try:
+            with ctx.prepare_batch([(Gc, "P11", 0, None)]) as B:
+                got, _ = B.run()
+            assert got == [O.count(a.rowptr, a.col, J.PATTERNS["P11"])]
+        finally:
+            Gc.free()
+    finally:
+        Ga.free()
+        Gb.free()

@@ -93,21 +93,29 @@ def test_heavy_lanes_on_fresh_graphs(ctx):
     equals the single-query count computed afterwards, with four lanes and
     with one."""
     from gen import hub_graph
-    gs = [ctx.upload(J.rmat(15, 16, seed=s)) for s in (7, 8)]
-    hub = ctx.upload(hub_graph(2600, 0.006, seed=26))
-    reqs = [(gs[0], "P1", None), (gs[1], "P3", None), (gs[0], "P7", None), (gs[1], "P13", None),
-            (gs[0], "P14", None), (gs[1], "P1", None), (gs[0], "P3", None), (hub, "TC", None),
-            (hub, "P13", None), (gs[1], "P4", (30000, 32768)), (gs[0], "P2", None), (gs[1], "P1", (50, 20000))]
+    csrs = [J.rmat(15, 16, seed=s) for s in (7, 8)] + [hub_graph(2600, 0.006, seed=26)]
+    spec = [(0, "P1", None), (1, "P3", None), (0, "P7", None), (1, "P13", None), (0, "P14", None),
+            (1, "P1", None), (0, "P3", None), (2, "TC", None), (2, "P13", None), (1, "P4", (30000, 32768)),
+            (0, "P2", None), (1, "P1", (50, 20000)), (1, "P11", None), (0, "P2", (100, 9000))]
     got = {}
-    try:
-        for lanes in (4, 1):
-            print(f"[case] serving {len(reqs)} heavy requests, {lanes} lane(s)", flush=True)
+    for lanes in (4, 1):
+        # fresh uploads each time: the scheduler builds every per-graph array
+        gs = [ctx.upload(c) for c in csrs]
+        try:
+            print(f"[case] serving {len(spec)} heavy requests on fresh graphs, {lanes} lane(s)", flush=True)
             with J.Server(ctx, heavy_lanes=lanes) as srv:
-                tickets = [srv.submit(G, p, r) for G, p, r in reqs]
+                tickets = [srv.submit(gs[i], p, r) for i, p, r in spec]
                 got[lanes] = [srv.wait(t)[0] for t in tickets]
-        alone = [G.count(p, r)[0] for G, p, r in reqs]
-        assert got[4] == alone
-        assert got[1] == alone
-    finally:
-        for G in gs + [hub]:
+        finally:
+            for G in gs:
+                G.free()
+    # alone: one query per fresh upload, so no array is shared with the runs above
+    alone = []
+    for i, p, r in spec:
+        G = ctx.upload(csrs[i])
+        try:
+            alone.append(G.count(p, r)[0])
+        finally:
             G.free()
+    assert got[4] == alone
+    assert got[1] == alone
