@@ -98,8 +98,16 @@ __global__ void __launch_bounds__(64 * BWPB) batch_kernel(BatchArgs a) {
     }
     const BQ Q = a.q[lo];
     const int64_t u0 = (k - a.cstart[lo]) * Q.chunk;
-    const int64_t u1 = min(Q.ntasks, u0 + Q.chunk);
+    int64_t u1 = min(Q.ntasks, u0 + Q.chunk);
     uint64_t acc = 0;
+    if (Q.kind == 102 && Q.tri) {
+      // diamonds from the per-edge triangle array: lanes over the edges
+      for (int64_t u = u0 + ln; u < u1; u += 64) {
+        const int64_t e = Q.base + u;
+        if (Q.ci[e] < Q.src[e]) acc += (uint64_t)symd::C2(Q.tri[e]);
+      }
+      u1 = u0;  // no per-edge pass
+    }
     for (int64_t u = u0; u < u1; ++u) {
       if (Q.kind < 100) {
         const int64_t v0 = Q.base + u;
@@ -230,7 +238,8 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
       b.chunk = 1;
     } else {
       const int p = kind - 100;
-      int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
+      // diamonds read the per-edge triangle array, as the single-query path does
+      int rc = ensure_sym_aux(g, p == 2 || pattern_uses_tri(p), p == 3, st);
       if (rc) return rc;
       b.rp = g->sym.rp;
       b.ci = g->sym.ci;
@@ -239,7 +248,7 @@ static int plan_batch(jgpm_ctx_s* c, const jgpm_query* q, int n, std::vector<jk:
       b.nsum = g->d_nsum;
       b.base = g->sym_rp_h[rb];
       b.ntasks = g->sym_rp_h[re] - b.base;
-      b.chunk = (p == 2 || p == 3) ? 8 * echunk : echunk;
+      b.chunk = p == 2 ? 64 * echunk : p == 3 ? 8 * echunk : echunk;
       max_sym = std::max(max_sym, g->sym_mmin);
     }
   }
