@@ -125,10 +125,13 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
     // reads it (a few bytes per edge)
     int rc = ensure_sym_aux(g, true, false, st);
     if (rc) return rc;
-    if (re <= rb) return JGPM_OK;
+    const int64_t e0 = g->sym_rp_h[rb], e1 = g->sym_rp_h[re];
+    if (e1 <= e0) return JGPM_OK;
+    const int64_t quads = (e1 - e0 + 3) / 4;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)8 * c->num_cu, (quads + 255) / 256));
     prof_begin(c, st, "sym_p2");
-    hipError_t e = jk::launch_vertex_sums(g->sym.rp, g->sym.ci, g->d_tri, rb, re, 2,
-                                          (unsigned long long*)(c->d_out + slot), grid_for(c, 4), st);
+    hipError_t e = jk::launch_p2_edges(g->d_src, g->sym.ci, g->d_tri, e0, e1, (unsigned long long*)(c->d_out + slot),
+                                       grid, st);
     prof_end(c, st);
     return e == hipSuccess ? JGPM_OK : JGPM_E_HIP;
   }
@@ -178,7 +181,7 @@ int enqueue_vertex_sum(jgpm_graph_s* g, int mode, int64_t rb, int64_t re, int sl
   if (rc) return rc;
   if (re <= rb) return JGPM_OK;
   jgpm_ctx_s* c = exec_ctx(g);
-  return jk::launch_vertex_sums(g->sym.rp, g->sym.ci, g->d_tri, rb, re, mode, (unsigned long long*)(c->d_out + slot),
+  return jk::launch_vertex_sums(g->sym.rp, g->d_tri, rb, re, mode, (unsigned long long*)(c->d_out + slot),
                                 grid_for(c, 4), st) == hipSuccess
              ? JGPM_OK
              : JGPM_E_HIP;

@@ -117,8 +117,11 @@ hipError_t launch_aux_tri(const int64_t* rp, const int32_t* ci, const int32_t* s
                           unsigned long long* qctr, int grid, hipStream_t st);
 hipError_t launch_aux_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int64_t* nsum, int grid, hipStream_t st);
 // mode 0: sum C(d,3) (3-stars); mode 1: sum t_v (d_v - 2) (tailed triangles)
-hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_t* tri, int64_t rb, int64_t re,
-                              int mode, unsigned long long* out, int grid, hipStream_t st);
+hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
+                              unsigned long long* out, int grid, hipStream_t st);
+// Diamonds over the directed edges [e0, e1) from the per-edge triangle array.
+hipError_t launch_p2_edges(const int32_t* src, const int32_t* ci, const int32_t* tri, int64_t e0, int64_t e1,
+                           unsigned long long* out, int grid, hipStream_t st);
 int tier_max_deg(int tier);
 size_t tier_scratch_words(int tier);
 

@@ -376,13 +376,10 @@ __global__ void k_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int64_t
   }
 }
 
-// Per-vertex pattern sums over the root range: 3-stars C(d,3) (mode 0),
-// tailed triangles t_v (d_v - 2) with t_v = (sum of tri over v's edges) / 2
-// (mode 1), and diamonds (P2, mode 2): the reference's per-edge value
-// C(|N(v0) ∩ N(v1)|, 2) for the edges (v0, v1), v1 < v0, of the roots v0,
-// read from the per-edge triangle array (a lower prefix of each list).
-__global__ void k_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_t* tri, int64_t rb, int64_t re,
-                              int mode, unsigned long long* out) {
+// Per-vertex pattern sums over the root range: 3-stars C(d,3) and tailed
+// triangles t_v (d_v - 2) with t_v = (sum of tri over v's edges) / 2.
+__global__ void k_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
+                              unsigned long long* out) {
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   uint64_t acc = 0;
@@ -390,11 +387,6 @@ __global__ void k_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_
     const int64_t d = rp[v + 1] - rp[v];
     if (mode == 0) {
       if (lane() == 0 && d >= 3) acc += (uint64_t)(d * (d - 1) / 2) * (uint64_t)(d - 2) / 3;
-    } else if (mode == 2) {
-      for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) {
-        if (ci[e] >= v) break;  // ascending list: the rest are above v
-        acc += (uint64_t)C2(tri[e]);
-      }
     } else {
       int64_t s = 0;
       for (int64_t e = rp[v] + lane(); e < rp[v + 1]; e += 64) s += tri[e];
@@ -404,6 +396,49 @@ __global__ void k_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_
   }
   acc = (uint64_t)wsum64((int64_t)acc);
   if (lane() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+// Diamonds (P2) over the directed edges [e0, e1) of a root range, from the
+// per-edge triangle array: sum of C(tri[e], 2) over the edges (v0, v1) with
+// v1 < v0, v0 = src[e].  One pass over three int32 streams, four edges per
+// thread per step (16-byte loads where the range allows), so the pass runs at
+// memory speed.
+__global__ void __launch_bounds__(256) k_p2_edges(const int32_t* __restrict__ src, const int32_t* __restrict__ ci,
+                                                  const int32_t* __restrict__ tri, int64_t e0, int64_t e1,
+                                                  unsigned long long* out) {
+  uint64_t acc = 0;
+  const int64_t nt = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // head up to the first multiple of 4, then aligned quads, then the tail
+  const int64_t a0 = min(e1, (e0 + 3) & ~(int64_t)3);
+  const int64_t a1 = max(a0, e1 & ~(int64_t)3);
+  for (int64_t e = e0 + tid; e < a0; e += nt)
+    if (ci[e] < src[e]) acc += (uint64_t)C2(tri[e]);
+  for (int64_t q = a0 / 4 + tid; q < a1 / 4; q += nt) {
+    const int4 s4 = reinterpret_cast<const int4*>(src)[q];
+    const int4 c4 = reinterpret_cast<const int4*>(ci)[q];
+    const int4 t4 = reinterpret_cast<const int4*>(tri)[q];
+    if (c4.x < s4.x) acc += (uint64_t)C2(t4.x);
+    if (c4.y < s4.y) acc += (uint64_t)C2(t4.y);
+    if (c4.z < s4.z) acc += (uint64_t)C2(t4.z);
+    if (c4.w < s4.w) acc += (uint64_t)C2(t4.w);
+  }
+  for (int64_t e = a1 + tid; e < e1; e += nt)
+    if (ci[e] < src[e]) acc += (uint64_t)C2(tri[e]);
+  acc = (uint64_t)wsum64((int64_t)acc);
+  __shared__ uint64_t part[4];
+  if (lane() == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t s = part[0] + part[1] + part[2] + part[3];
+    if (s) atomicAdd(out, (unsigned long long)s);
+  }
+}
+
+hipError_t launch_p2_edges(const int32_t* src, const int32_t* ci, const int32_t* tri, int64_t e0, int64_t e1,
+                           unsigned long long* out, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_p2_edges, dim3(grid), dim3(256), 0, st, src, ci, tri, e0, e1, out);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -446,9 +481,9 @@ hipError_t launch_aux_nsum(const int64_t* rp, const int32_t* ci, int32_t nv, int
   hipLaunchKernelGGL(k_nsum, dim3(grid), dim3(256), 0, st, rp, ci, nv, nsum);
   return hipGetLastError();
 }
-hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* ci, const int32_t* tri, int64_t rb, int64_t re,
-                              int mode, unsigned long long* out, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_vertex_sums, dim3(grid), dim3(256), 0, st, rp, ci, tri, rb, re, mode, out);
+hipError_t launch_vertex_sums(const int64_t* rp, const int32_t* tri, int64_t rb, int64_t re, int mode,
+                              unsigned long long* out, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_vertex_sums, dim3(grid), dim3(256), 0, st, rp, tri, rb, re, mode, out);
   return hipGetLastError();
 }
 
