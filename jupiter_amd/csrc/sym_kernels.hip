@@ -112,13 +112,17 @@ constexpr uint64_t P1_EMPTY = 0xffffffff00000000ull;  // key -1, count 0
 // v0 start at ci[b0]), 64 neighbours at a time.
 template <class F>
 __device__ void p1_wedges(const int64_t* rp, const int32_t* ci, int64_t b0, int m0, int32_t v0, int32_t* off,
-                          int64_t* lbase, F f) {
+                          int64_t* lbase, F f, int len0 = -1, int64_t base0 = 0) {
   const int ln = lane();
   for (int jb = 0; jb < m0; jb += 64) {
     const int j = jb + ln;
     int len = 0;
     int64_t base = 0;
-    if (j < m0) {
+    if (jb == 0 && len0 >= 0) {
+      // the first 64 neighbours' segments, found by the caller
+      len = len0;
+      base = base0;
+    } else if (j < m0) {
       const int32_t v = ci[b0 + j];
       base = rp[v];
       len = lb(ci + base, (int)(rp[v + 1] - base), v0);  // x < v0
@@ -156,12 +160,21 @@ __device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, ui
   const int ln = lane();
   const int64_t b0 = rp[v0];
   const int m0 = lb(ci + b0, (int)(rp[v0 + 1] - b0), v0);  // neighbours below v0
-  // wedge count of the root, to pick the table
+  // wedge count of the root, to pick the table; the first 64 neighbours'
+  // segments are kept for the wedge passes (most roots have no more), so
+  // their binary searches are not repeated
   int64_t w = 0;
+  int len0 = 0;
+  int64_t base0 = 0;
   for (int j = ln; j < m0; j += 64) {
     const int32_t v = ci[b0 + j];
     const int64_t base = rp[v];
-    w += lb(ci + base, (int)(rp[v + 1] - base), v0);
+    const int len = lb(ci + base, (int)(rp[v + 1] - base), v0);
+    if (j == ln) {
+      len0 = len;
+      base0 = base;
+    }
+    w += len;
   }
   w = wsum64(w);
   uint64_t acc = 0;
@@ -191,7 +204,7 @@ __device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, ui
         }
         h = (h + 1) & (used - 1);
       }
-    });
+    }, len0, base0);
     __syncwarp();
     for (int k = ln; k < used; k += 64) {
       const uint64_t v = tab[k];
@@ -231,7 +244,7 @@ __device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, ui
           }
           h = (h + 1) & (uint64_t)(used - 1);
         }
-      });
+      }, len0, base0);
       __threadfence();
       for (int64_t k = ln; k < used; k += 64) {
         const uint64_t v = atomicExch((unsigned long long*)&gt[k], 0ull);
@@ -240,12 +253,12 @@ __device__ uint64_t p1_root(const int64_t* rp, const int32_t* ci, int32_t v0, ui
       }
       __threadfence();
     } else {
-      p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) { atomicAdd(&cnt[x], 1); });
+      p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) { atomicAdd(&cnt[x], 1); }, len0, base0);
       __threadfence();  // the increments are performed before they are taken back
       p1_wedges(rp, ci, b0, m0, v0, off, lbase, [&](int32_t x) {
         const uint64_t c = (uint32_t)atomicExch(&cnt[x], 0);
         acc += c * (c - 1) / 2;
-      });
+      }, len0, base0);
       __threadfence();
     }
   }
