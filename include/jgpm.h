@@ -174,6 +174,10 @@ typedef struct {
                                whose per-edge work is O(1) after the intersection, use 8x) */
   int32_t batch_blocks_per_cu; /* persistent batched grid = this * num_cu, capped by the
                                   kernel's occupancy                                       */
+  int32_t sym_heavy_min;    /* P5, P7, P9, P11, P12 (single-query launches): directed edges
+                               with more than this many common neighbours (up to 8192) run
+                               one block per edge, S in an LDS hash (default 32; 8192
+                               turns it off)                                               */
 } jgpm_tuning;
 int  jgpm_get_tuning(jgpm_ctx_t ctx, jgpm_tuning* t);
 int  jgpm_set_tuning(jgpm_ctx_t ctx, const jgpm_tuning* t);

@@ -59,7 +59,7 @@ class _Tuning(ctypes.Structure):
                 ("gallop_ratio", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("batch_hub_deg", ctypes.c_int32),
                 ("batch_root_chunk", ctypes.c_int32), ("batch_edge_chunk", ctypes.c_int32),
-                ("batch_blocks_per_cu", ctypes.c_int32)]
+                ("batch_blocks_per_cu", ctypes.c_int32), ("sym_heavy_min", ctypes.c_int32)]
 
 
 class _Query(ctypes.Structure):

@@ -53,6 +53,14 @@ int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t s
 
 bool pattern_uses_tri(int p) { return p == 3 || p == 4 || p == 7 || p == 10 || p == 11; }
 
+// The single-query launches of P5, P7, P9, P11 and P12 read the triangle
+// array to send their heavy edges to the block-per-edge kernel.
+static bool heavy_split_on(const jgpm_ctx_s* c, int p) {
+  return jk::sym_heavy_pattern(p) && c->tune.sym_heavy_min < jk::sym_heavy_nmax();
+}
+
+bool query_needs_tri(const jgpm_ctx_s* c, int p) { return p == 2 || pattern_uses_tri(p) || heavy_split_on(c, p); }
+
 // Waves of the 4-cycle wedge kernel on a graph of nv vertices: one dense
 // counter array of nv ints each, within 2 GiB, at most 32 waves per CU.
 static int64_t p1_waves(jgpm_ctx_s* c, int32_t nv) {
@@ -137,7 +145,8 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   }
   // The triangle array is shared by later queries; its counter slot is
   // reset per query by ctx_reset_counters before it is used here.
-  int rc = ensure_sym_aux(g, pattern_uses_tri(p), p == 3, st);
+  const bool heavy = heavy_split_on(c, p);
+  int rc = ensure_sym_aux(g, query_needs_tri(c, p), p == 3, st);
   if (rc) return rc;
   const int wpb = jk::sym_waves_per_block();
   const int grid = grid_for(c, 2);
@@ -167,11 +176,33 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   a.scr = c->d_sscr;
   a.scr_stride = (int64_t)stride;
   a.out = c->d_out + slot;
+  a.heavy_min = heavy ? c->tune.sym_heavy_min : 0;
+  a.heavy_stream = jk::sym_heavy_stream();
   static const char* const kSymName[13] = {"", "sym_p1", "sym_p2", "sym_p3", "sym_p4", "sym_p5", "sym_p6",
                                            "sym_p7", "sym_p8", "sym_p9", "sym_p10", "sym_p11", "sym_p12"};
   prof_begin(c, st, kSymName[p]);
   hipError_t e = jk::launch_sym(p, a, grid, st);
   prof_end(c, st);
+  if (e != hipSuccess) return JGPM_E_HIP;
+  if (heavy) {
+    // the edges left by the wave-per-edge launch, with its queue counter
+    // taken back to zero (a kernel, so the sequence can be captured)
+    // in two shapes: |S| up to sym_heavy_mid() with a small LDS table and
+    // several blocks per CU, wider ones one block per CU
+    static const char* const kHeavyName[13] = {"", "", "", "", "", "sym_p5_heavy", "", "sym_p7_heavy", "",
+                                               "sym_p9_heavy", "", "sym_p11_heavy", "sym_p12_heavy"};
+    const int mid = jk::sym_heavy_mid();
+    for (int wide = 0; wide < 2 && e == hipSuccess; ++wide) {
+      jk::SymArgs h = a;
+      h.heavy_min = wide ? std::max(a.heavy_min, mid) : a.heavy_min;
+      h.heavy_max = wide ? jk::sym_heavy_nmax() : mid;
+      if (h.heavy_min >= h.heavy_max) continue;
+      if (zero_u64(reinterpret_cast<uint64_t*>(a.qctr), 1, st)) return JGPM_E_HIP;
+      prof_begin(c, st, kHeavyName[p]);
+      e = jk::launch_sym_heavy(p, h, wide != 0, wide ? c->num_cu : 4 * c->num_cu, st);
+      prof_end(c, st);
+    }
+  }
   return e == hipSuccess ? JGPM_OK : JGPM_E_HIP;
 }
 
@@ -245,8 +276,7 @@ int prepare_pattern(jgpm_graph_s* g, int pattern, int64_t rb, int64_t re) {
   } else if (pattern == JGPM_P1) {
     if (g->has_sym && rb == 0 && re == g->sym.nv) rc = ensure_rank_csr(g, c->st);
   } else if (pattern >= JGPM_P2 && pattern <= JGPM_P12) {
-    if (g->has_sym)
-      rc = ensure_sym_aux(g, pattern == JGPM_P2 || pattern_uses_tri(pattern), pattern == JGPM_P3, c->st);
+    if (g->has_sym) rc = ensure_sym_aux(g, query_needs_tri(c, pattern), pattern == JGPM_P3, c->st);
   }
   if (hipStreamSynchronize(c->st) != hipSuccess && !rc) rc = JGPM_E_HIP;
   return rc;

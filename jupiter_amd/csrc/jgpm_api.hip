@@ -242,6 +242,7 @@ static jgpm_tuning default_tuning() {
   t.batch_root_chunk = 16;
   t.batch_edge_chunk = 8;
   t.batch_blocks_per_cu = 0;  // 0: the batched kernel's occupancy
+  t.sym_heavy_min = 32;
   return t;
 }
 
@@ -343,6 +344,7 @@ extern "C" int jgpm_set_tuning(jgpm_ctx_t c, const jgpm_tuning* t) {
   put(t->batch_root_chunk, d.batch_root_chunk, c->tune.batch_root_chunk, 1, 4096);
   put(t->batch_edge_chunk, d.batch_edge_chunk, c->tune.batch_edge_chunk, 1, 4096);
   put(t->batch_blocks_per_cu, d.batch_blocks_per_cu, c->tune.batch_blocks_per_cu, 1, 64);
+  put(t->sym_heavy_min, d.sym_heavy_min, c->tune.sym_heavy_min, 1, 1 << 20);
   return JGPM_OK;
 }
 

@@ -140,6 +140,8 @@ int ensure_sym_aux(jgpm_graph_s* g, bool need_tri, bool need_nsum, hipStream_t s
 int enqueue_batch_q(jgpm_ctx_s* c, const jgpm_query* q, int n, hipStream_t st);
 bool batch_supported(int pattern);
 bool pattern_uses_tri(int p);
+// whether a single-query launch of pattern p on context c reads the triangle array
+bool query_needs_tri(const jgpm_ctx_s* c, int p);
 // profile hooks around one kernel launch on the context stream
 void prof_begin(jgpm_ctx_s* c, hipStream_t st, const char* name);
 void prof_end(jgpm_ctx_s* c, hipStream_t st);

@@ -65,8 +65,20 @@ struct SymArgs {
   int32_t* scr;          // per-wave list scratch for lists longer than the LDS capacity
   int64_t scr_stride;    // ints per scratch list (>= max degree)
   uint64_t* out;
+  // P5, P7, P9, P11, P12: edges with heavy_min < tri[e] <= sym_heavy_nmax()
+  // are left to launch_sym_heavy (0: every edge runs here; needs tri)
+  int32_t heavy_min;
+  int32_t heavy_max;     // (launch_sym_heavy) edges with heavy_min < tri[e] <= heavy_max
+  int32_t heavy_stream;  // stream N(v2) when d(v2) <= heavy_stream * |S|, else binary searches
 };
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
+// The heavy edges of P5, P7, P9, P11 and P12, one block per edge (a.qctr
+// zero before; edges taken 64 at a time); wide: |S| above sym_heavy_mid().
+bool sym_heavy_pattern(int p);
+int sym_heavy_nmax();
+int sym_heavy_stream();
+int sym_heavy_mid();  // |S| up to this: the narrow launch (more blocks per CU)
+hipError_t launch_sym_heavy(int p, const SymArgs& a, bool wide, int grid, hipStream_t st);
 
 // 4-cycles (P1) by wedges: one wave per root of [rb, re), counters in a
 // per-wave dense array of nv ints (cnt + wave * nv), zero before and after.

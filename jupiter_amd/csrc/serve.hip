@@ -124,7 +124,7 @@ static int64_t footprint(jgpm_graph_s* g, int p, const jgpm_ctx_s* c, int grid) 
   int64_t b = 64;  // count slot and queue counters
   if (!is_dag_pattern(p)) {
     if (!g->d_src) b += 4 * g->sym.nnz;
-    if (!g->d_tri && (p == JGPM_P2 || jgpm::pattern_uses_tri(p))) b += 4 * g->sym.nnz;
+    if (!g->d_tri && jgpm::query_needs_tri(c, p)) b += 4 * g->sym.nnz;
     if (!g->d_nsum && p == JGPM_P3) b += 8 * (int64_t)g->sym.nv;
     b += (int64_t)grid * 4 * 5 * 4 * std::max(1, g->sym_mmin);  // list scratch of the batch kernel
   } else if (g->dag.max_deg > jk::tier_max_deg(jk::TIER_LDS16)) {

@@ -12,6 +12,16 @@ constexpr int WPB = 4;        // waves per block, each an independent edge worke
 constexpr int CAP = 256;      // LDS list capacity per wave (ints)
 constexpr int STAGE = 256;    // longest search list staged into LDS
 constexpr int NLIST = 5;      // per-wave lists: S, posS0, posS1, L, posL0
+// block-per-edge kernel for the heavy edges of the q-sum patterns
+constexpr int HV_WAVES = 16;           // waves per block (one edge)
+constexpr int HV_NMAX = 8192;          // largest |S| it takes
+constexpr int HV_NMID = 1024;          // |S| up to this: the narrow shape
+constexpr int HV_WAVES_MID = 8;
+// stream N(v2) when d(v2) <= HV_STREAM * |S|, else binary searches (R-MAT 18
+// P9: ratio 2 / 4 / 8 / 16 / 32 / 64 -> 274 / 233 / 227 / 250 / 291 / 357 ms)
+constexpr int HV_STREAM = 8;
+template <int P>
+constexpr bool heavy_split() { return P == 5 || P == 7 || P == 9 || P == 11 || P == 12; }
 
 __device__ __forceinline__ int lane() { return (int)__lane_id(); }
 

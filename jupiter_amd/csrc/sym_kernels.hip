@@ -65,6 +65,12 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
     const int64_t end = min(a.ne, base + (int64_t)a.chunk);
     for (int64_t e = base; e < end; ++e) {
       const int64_t eid = a.e0 + e;
+      if constexpr (heavy_split<P>()) {
+        if (a.heavy_min > 0) {  // left to sym_heavy_kernel
+          const int32_t t = a.tri[eid];
+          if (t > a.heavy_min && t <= HV_NMAX) continue;
+        }
+      }
       const int32_t v0 = a.src[eid];
       const int32_t v1 = a.ci[eid];
       // lists fit in LDS when the smaller endpoint degree does
@@ -84,6 +90,180 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
     for (int k = 0; k < WPB; ++k) s += s_part[k];
     if (s) atomicAdd((unsigned long long*)a.out, (unsigned long long)s);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Heavy edges of P5, P7, P9, P11 and P12.  Their per-edge value sums terms
+// of q(v2) = |S ∩ N(v2)| over the members v2 of S = N(v0) ∩ N(v1), and on
+// skewed graphs nearly all the work sits in the few edges with a large S
+// (R-MAT 18: the 12 % of directed edges with |S| > 128 hold 93 % of sum |S|^2).  A
+// wave per edge computes each q(v2) as |S| binary searches in N(v2).  Here a
+// block takes the edge: it intersects the two lists once, keeps S in LDS
+// both as a list and as an open-addressing hash set, and its waves take the
+// members v2 one at a time, counting q(v2) by streaming N(v2) (coalesced
+// loads, one LDS probe sequence per element) when d(v2) <= HV_STREAM * |S|,
+// else by binary searches of S's members in N(v2).  The per-edge value is the
+// one edge_value<P> returns (the same formulas on the same q, b and c).
+template <int P>
+__device__ __forceinline__ bool heavy_orient(int32_t v0, int32_t v1) {
+  if constexpr (P == 5) return v1 < v0;
+  else if constexpr (P == 7) return true;
+  else return v1 > v0;  // P9, P11, P12
+}
+
+__device__ __forceinline__ uint32_t hv_hash(int32_t x, int lg) { return ((uint32_t)x * 2654435761u) >> (32 - lg); }
+
+__device__ __forceinline__ int hv_has(const int32_t* tab, int mask, int lg, int32_t x) {
+  uint32_t h = hv_hash(x, lg);
+  for (;;) {
+    const int32_t k = tab[h];
+    if (k == x) return 1;
+    if (k < 0) return 0;
+    h = (h + 1) & (uint32_t)mask;
+  }
+}
+
+// Two shapes: |S| <= NMAX of 1024 with 8 waves (small LDS, several blocks per
+// CU) and of 8192 with 16 waves (one block per CU); the launch takes the
+// edges with heavy_min < |S| <= heavy_max.
+template <int P, int NMAX, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
+  constexpr int TMAX = NMAX <= 1024 ? 4 * NMAX : 2 * NMAX;  // hash slots
+  __shared__ int32_t s_S[NMAX];
+  __shared__ int32_t s_tab[TMAX];  // vertex ids of S, -1 = empty
+  __shared__ int64_t s_red[WAVES];
+  __shared__ int64_t s_base;
+  __shared__ uint64_t s_mask;
+  __shared__ int32_t s_n, s_next;
+  constexpr int NT = 64 * WAVES;
+  const int tid = threadIdx.x;
+  const int wv = tid >> 6;
+  const int ln = lane();
+  for (int k = tid; k < TMAX; k += NT) s_tab[k] = -1;
+  uint64_t acc = 0;  // thread 0's
+  for (;;) {
+    if (tid == 0) s_base = (int64_t)atomicAdd(a.qctr, 64ull);
+    __syncthreads();
+    const int64_t base = s_base;
+    if (base >= a.ne) break;
+    if (wv == 0) {
+      bool h = false;
+      if (base + ln < a.ne) {
+        const int64_t eid = a.e0 + base + ln;
+        const int32_t t = a.tri[eid];
+        h = t > a.heavy_min && t <= a.heavy_max && t <= NMAX && heavy_orient<P>(a.src[eid], a.ci[eid]);
+      }
+      const uint64_t m = __ballot(h);
+      if (ln == 0) s_mask = m;
+    }
+    __syncthreads();
+    for (uint64_t m = s_mask; m; m &= m - 1) {
+      const int64_t eid = a.e0 + base + __builtin_ctzll(m);
+      const int32_t v0 = a.src[eid], v1 = a.ci[eid];
+      const int n = a.tri[eid];
+      const int64_t b0 = a.rp[v0], b1 = a.rp[v1];
+      const int d0 = (int)(a.rp[v0 + 1] - b0), d1 = (int)(a.rp[v1 + 1] - b1);
+      const int32_t* N0 = a.ci + b0;
+      const int32_t* N1 = a.ci + b1;
+      // load <= 1/4 while the table allows (a miss, the common case, then
+      // ends after ~1.4 probes), else <= 1/2
+      int lg = 6;
+      while ((1 << lg) < 4 * n && (1 << lg) < TMAX) ++lg;
+      const int mask = (1 << lg) - 1;
+      if (tid == 0) {
+        s_n = 0;
+        s_next = 0;
+      }
+      __syncthreads();
+      // S: the shorter list's elements searched in the longer one
+      const int32_t* A = d0 <= d1 ? N0 : N1;
+      const int32_t* B = d0 <= d1 ? N1 : N0;
+      const int na = min(d0, d1), nb = max(d0, d1);
+      for (int i0 = 0; i0 < na; i0 += NT) {
+        const int i = i0 + tid;
+        int32_t x = 0;
+        bool f = false;
+        if (i < na) {
+          x = A[i];
+          const int p = lb(B, nb, x);
+          f = p < nb && B[p] == x;
+        }
+        const uint64_t mk = __ballot(f);
+        int kb = 0;
+        if (ln == 0 && mk) kb = atomicAdd(&s_n, (int)__popcll(mk));
+        kb = __shfl(kb, 0);
+        const int k = kb + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+        if (f && k < n) {  // k < n always: the triangle array holds |S|
+          s_S[k] = x;
+          uint32_t h = hv_hash(x, lg);
+          while (atomicCAS(&s_tab[h], -1, x) != -1) h = (h + 1) & (uint32_t)mask;
+        }
+      }
+      __syncthreads();
+      // q(v2) for the members, one wave per member at a time
+      int64_t part = 0;  // lane 0's
+      for (;;) {
+        int j = 0;
+        if (ln == 0) j = atomicAdd(&s_next, 1);
+        j = __shfl(j, 0);
+        if (j >= n) break;
+        const int32_t v2 = s_S[j];
+        if constexpr (P == 7 || P == 9 || P == 11) {
+          if (v2 <= v1) continue;
+        }
+        int64_t r2 = a.rp[v2];
+        int d2 = (int)(a.rp[v2 + 1] - r2);
+        if constexpr (P == 5) {
+          // P5 needs only sum_S q = 2 |E(S)|: each edge of S is taken once,
+          // from its smaller end, so only N(v2)'s members above v2
+          const int lo = lb(a.ci + r2, d2, v2 + 1);
+          r2 += lo;
+          d2 -= lo;
+        }
+        const int32_t* N2 = a.ci + r2;
+        int q = 0;
+        if (d2 <= a.heavy_stream * n) {
+          int k = ln;
+          for (; k + 192 < d2; k += 256) {  // four loads in flight
+            const int32_t x0 = N2[k], x1 = N2[k + 64], x2 = N2[k + 128], x3 = N2[k + 192];
+            q += hv_has(s_tab, mask, lg, x0) + hv_has(s_tab, mask, lg, x1) + hv_has(s_tab, mask, lg, x2) +
+                 hv_has(s_tab, mask, lg, x3);
+          }
+          for (; k < d2; k += 64) q += hv_has(s_tab, mask, lg, N2[k]);
+        } else {
+          for (int k = ln; k < n; k += 64) q += in_sorted(N2, d2, s_S[k]);  // (P5: members > v2 only)
+        }
+        q = wsum(q);
+        if (ln == 0) {
+          if constexpr (P == 5) {
+            part += q;
+          } else if constexpr (P == 9 || P == 12) {
+            part += C2(q);
+          } else if constexpr (P == 7) {
+            part += (int64_t)(n - 1) * (a.tri[b0 + lb(N0, d0, v2)] - 1) - q;
+          } else {  // P11
+            const int64_t aa = n - 1;
+            const int64_t bb = a.tri[b0 + lb(N0, d0, v2)] - 1;
+            const int64_t cc = a.tri[b1 + lb(N1, d1, v2)] - 1;
+            part += aa * bb * cc - (int64_t)q * (aa + bb + cc) + 2 * (int64_t)q;
+          }
+        }
+      }
+      if (ln == 0) s_red[wv] = part;
+      __syncthreads();
+      if (tid == 0) {
+        int64_t c = 0;
+        for (int k = 0; k < WAVES; ++k) c += s_red[k];
+        if constexpr (P == 5) c = n >= 3 ? (int64_t)(n - 2) * c : 0;  // c = |E(S)|
+        else if constexpr (P == 12) c = n >= 3 ? (int64_t)(n - 3) * c : 0;
+        acc += (uint64_t)c;
+      }
+      for (int k = tid; k <= mask; k += NT) s_tab[k] = -1;
+      __syncthreads();
+    }
+  }
+  if (tid == 0 && acc) atomicAdd((unsigned long long*)a.out, (unsigned long long)acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -474,6 +654,30 @@ hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st) {
     case 10: return launch_p<10>(a, grid, st);
     case 11: return launch_p<11>(a, grid, st);
     case 12: return launch_p<12>(a, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int P>
+static hipError_t launch_heavy_p(const SymArgs& a, bool wide, int grid, hipStream_t st) {
+  if (wide) hipLaunchKernelGGL((sym_heavy_kernel<P, HV_NMAX, HV_WAVES>), dim3(grid), dim3(64 * HV_WAVES), 0, st, a);
+  else hipLaunchKernelGGL((sym_heavy_kernel<P, HV_NMID, HV_WAVES_MID>), dim3(grid), dim3(64 * HV_WAVES_MID), 0, st, a);
+  return hipGetLastError();
+}
+
+bool sym_heavy_pattern(int p) { return p == 5 || p == 7 || p == 9 || p == 11 || p == 12; }
+int sym_heavy_nmax() { return HV_NMAX; }
+int sym_heavy_stream() { return HV_STREAM; }
+
+int sym_heavy_mid() { return HV_NMID; }
+
+hipError_t launch_sym_heavy(int p, const SymArgs& a, bool wide, int grid, hipStream_t st) {
+  switch (p) {
+    case 5: return launch_heavy_p<5>(a, wide, grid, st);
+    case 7: return launch_heavy_p<7>(a, wide, grid, st);
+    case 9: return launch_heavy_p<9>(a, wide, grid, st);
+    case 11: return launch_heavy_p<11>(a, wide, grid, st);
+    case 12: return launch_heavy_p<12>(a, wide, grid, st);
   }
   return hipErrorInvalidValue;
 }

@@ -190,3 +190,91 @@ def test_dag_only_graph_rejects_sym_patterns(ctx):
         G.count("P1")
     assert e.value.status == -7
     G.free()
+
+
+HEAVY = ["P5", "P7", "P9", "P11", "P12"]
+
+
+def test_heavy_edge_kernel_thresholds(ctx):
+    """P5, P7, P9, P11 and P12 send the edges with more than sym_heavy_min
+    common neighbours to the block-per-edge kernel (S in an LDS hash; q(v2)
+    by streaming N(v2) or by binary searches, whichever is shorter).  At
+    every threshold, from every edge heavy (1) to none (8192), the counts
+    equal the oracle's golden counts and quarters on R-MAT 12, the oracle on
+    a hub graph, and the closed form on K40 (every edge has 38 common
+    neighbours)."""
+    ent = GOLD["rmat12_s7"]
+    g12 = J.rmat(12, 16, seed=7)
+    hub = hub_graph(300, 0.08, seed=11)
+    src, dst = np.triu_indices(40, 1)
+    k40 = J.CSR.from_edges(40, src, dst)
+    want_hub = {p: O.count(hub.rowptr, hub.col, J.PATTERNS[p]) for p in HEAVY}
+    want_k40 = {}
+    for p in HEAVY:
+        k, edges = brute.PATTERNS[J.PATTERNS[p]]
+        want_k40[p] = comb(40, k) * int(np.prod(range(1, k + 1))) // brute.automorphisms(k, edges)
+    G12, Gh, Gk = ctx.upload(g12), ctx.upload(hub), ctx.upload(k40)
+    q = g12.nv // 4
+    try:
+        for h in [1, 8, 32, 200, 8192]:
+            ctx.set_tuning(sym_heavy_min=h)
+            assert ctx.tuning()["sym_heavy_min"] == h
+            for p in HEAVY:
+                _say(f"heavy_min {h} {p}")
+                assert G12.count(p)[0] == ent["counts"][p], (h, p)
+                assert [G12.count(p, (i * q, (i + 1) * q))[0] for i in range(4)] == ent["quarters"][p], (h, p)
+                assert Gh.count(p)[0] == want_hub[p], (h, p)
+                assert Gk.count(p)[0] == want_k40[p], (h, p)
+    finally:
+        ctx.set_tuning(sym_heavy_min=-1)
+    assert ctx.tuning()["sym_heavy_min"] == 32
+    for G in (G12, Gh, Gk):
+        G.free()
+
+
+def test_heavy_edge_kernel_wide_shape(ctx):
+    """An edge whose endpoints share 1,200 neighbours (two hubs over a sparse
+    random graph) runs in the 8192-wide shape of the heavy kernel, the
+    others in the 1024-wide one or the wave-per-edge kernel: equal to the
+    oracle at every threshold."""
+    rng = np.random.default_rng(3)
+    m = 1200
+    com = np.arange(2, m + 2)
+    iu, ju = np.triu_indices(m, 1)
+    keep = rng.random(len(iu)) < 0.01
+    src = np.concatenate([[0], np.zeros(m, np.int64), np.ones(m, np.int64), iu[keep] + 2])
+    dst = np.concatenate([[1], com, com, ju[keep] + 2])
+    g = J.CSR.from_edges(m + 2, src, dst)
+    G = ctx.upload(g)
+    want = {p: O.count(g.rowptr, g.col, J.PATTERNS[p]) for p in HEAVY}
+    want2 = {p: O.count(g.rowptr, g.col, J.PATTERNS[p], 0, 2) for p in HEAVY}  # the two hubs
+    try:
+        for h in [1, 32, 1000, 8192]:
+            ctx.set_tuning(sym_heavy_min=h)
+            for p in HEAVY:
+                _say(f"two hubs heavy_min {h} {p}")
+                assert G.count(p)[0] == want[p], (h, p)
+                assert G.count(p, (0, 2))[0] == want2[p], (h, p)
+    finally:
+        ctx.set_tuning(sym_heavy_min=-1)
+    G.free()
+
+
+def test_heavy_edge_kernel_rmat16(ctx):
+    """On R-MAT 16 (edges with up to ~1,000 common neighbours) the split run
+    at the default threshold equals the wave-per-edge run over the whole
+    graph and over a window of roots."""
+    g = J.rmat(16, 16, seed=5)
+    G = ctx.upload(g)
+    win = (100, g.nv // 3)
+    try:
+        for p in HEAVY:
+            _say(f"rmat16 heavy {p}")
+            ctx.set_tuning(sym_heavy_min=8192)
+            off = (G.count(p)[0], G.count(p, win)[0])
+            ctx.set_tuning(sym_heavy_min=-1)
+            on = (G.count(p)[0], G.count(p, win)[0])
+            assert on == off, p
+    finally:
+        ctx.set_tuning(sym_heavy_min=-1)
+    G.free()
