@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
         if constexpr (P == 5) {
           // P5 needs only sum_S q = 2 |E(S)|: each edge of S is taken once,
           // from its smaller end, so only N(v2)'s members above v2
-          const int lo = lb(a.ci + r2, d2, v2 + 1);
+          const int lo = w_lb(a.ci + r2, d2, v2 + 1);
           r2 += lo;
           d2 -= lo;
         }
@@ -235,17 +235,19 @@ __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
           for (int k = ln; k < n; k += 64) q += in_sorted(N2, d2, s_S[k]);  // (P5: members > v2 only)
         }
         q = wsum(q);
+        // t(v0, v2) and t(v1, v2) from v2's positions in N(v0) and N(v1)
+        int64_t bb = 0, cc = 0;
+        if constexpr (P == 7 || P == 11) bb = a.tri[b0 + w_lb(N0, d0, v2)] - 1;
+        if constexpr (P == 11) cc = a.tri[b1 + w_lb(N1, d1, v2)] - 1;
         if (ln == 0) {
           if constexpr (P == 5) {
             part += q;
           } else if constexpr (P == 9 || P == 12) {
             part += C2(q);
           } else if constexpr (P == 7) {
-            part += (int64_t)(n - 1) * (a.tri[b0 + lb(N0, d0, v2)] - 1) - q;
+            part += (int64_t)(n - 1) * bb - q;
           } else {  // P11
             const int64_t aa = n - 1;
-            const int64_t bb = a.tri[b0 + lb(N0, d0, v2)] - 1;
-            const int64_t cc = a.tri[b1 + lb(N1, d1, v2)] - 1;
             part += aa * bb * cc - (int64_t)q * (aa + bb + cc) + 2 * (int64_t)q;
           }
         }
