@@ -145,9 +145,20 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   }
   // The triangle array is shared by later queries; its counter slot is
   // reset per query by ctx_reset_counters before it is used here.
-  const bool heavy = heavy_split_on(c, p);
   int rc = ensure_sym_aux(g, query_needs_tri(c, p), p == 3, st);
   if (rc) return rc;
+  // the heavy-edge split needs its two edge lists (allocated once per
+  // context, so not while a launch sequence is being captured) and edge
+  // indices of a range that fit 32 bits
+  bool heavy = heavy_split_on(c, p) && g->sym_rp_h[re] - g->sym_rp_h[rb] < INT32_MAX;
+  if (heavy && !c->d_hlist) {
+    if (c->capturing) {
+      heavy = false;
+    } else {
+      if (hipMalloc(&c->d_hlist, sizeof(int32_t) * 2 * (size_t)jk::sym_heavy_cap()) != hipSuccess) return JGPM_E_NOMEM;
+      if (hipMalloc(&c->d_hctr, sizeof(unsigned long long) * 4) != hipSuccess) return JGPM_E_NOMEM;
+    }
+  }
   const int wpb = jk::sym_waves_per_block();
   const int grid = grid_for(c, 2);
   const size_t stride = (size_t)std::max(1, g->sym_mmin);
@@ -178,6 +189,13 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   a.out = c->d_out + slot;
   a.heavy_min = heavy ? c->tune.sym_heavy_min : 0;
   a.heavy_stream = jk::sym_heavy_stream();
+  a.nv = g->sym.nv;
+  a.hlist = c->d_hlist;
+  a.hctr = c->d_hctr;
+  a.hcap = jk::sym_heavy_cap();
+  a.heavy_mid = heavy ? jk::sym_heavy_mid(p) : 0;
+  // list counters to zero (a kernel, so the sequence can be captured)
+  if (heavy && zero_u64(reinterpret_cast<uint64_t*>(c->d_hctr), 4, st)) return JGPM_E_HIP;
   static const char* const kSymName[13] = {"", "sym_p1", "sym_p2", "sym_p3", "sym_p4", "sym_p5", "sym_p6",
                                            "sym_p7", "sym_p8", "sym_p9", "sym_p10", "sym_p11", "sym_p12"};
   prof_begin(c, st, kSymName[p]);
@@ -185,21 +203,14 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   prof_end(c, st);
   if (e != hipSuccess) return JGPM_E_HIP;
   if (heavy) {
-    // the edges left by the wave-per-edge launch, with its queue counter
-    // taken back to zero (a kernel, so the sequence can be captured)
-    // in two shapes: |S| up to sym_heavy_mid() with a small LDS table and
-    // several blocks per CU, wider ones one block per CU
+    // the edges the wave-per-edge launch listed, in two shapes: |S| up to
+    // sym_heavy_mid(p) with a small LDS table and several blocks per CU,
+    // wider ones one block per CU
     static const char* const kHeavyName[13] = {"", "", "", "", "", "sym_p5_heavy", "", "sym_p7_heavy", "",
                                                "sym_p9_heavy", "", "sym_p11_heavy", "sym_p12_heavy"};
-    const int mid = jk::sym_heavy_mid();
     for (int wide = 0; wide < 2 && e == hipSuccess; ++wide) {
-      jk::SymArgs h = a;
-      h.heavy_min = wide ? std::max(a.heavy_min, mid) : a.heavy_min;
-      h.heavy_max = wide ? jk::sym_heavy_nmax() : mid;
-      if (h.heavy_min >= h.heavy_max) continue;
-      if (zero_u64(reinterpret_cast<uint64_t*>(a.qctr), 1, st)) return JGPM_E_HIP;
       prof_begin(c, st, kHeavyName[p]);
-      e = jk::launch_sym_heavy(p, h, wide != 0, wide ? c->num_cu : 4 * c->num_cu, st);
+      e = jk::launch_sym_heavy(p, a, wide != 0, wide ? c->num_cu : 4 * c->num_cu, st);
       prof_end(c, st);
     }
   }

@@ -297,6 +297,8 @@ extern "C" int jgpm_shutdown(jgpm_ctx_t c) {
   if (c->d_q) (void)hipFree(c->d_q);
   if (c->d_q64) (void)hipFree(c->d_q64);
   if (c->d_sscr) (void)hipFree(c->d_sscr);
+  if (c->d_hlist) (void)hipFree(c->d_hlist);
+  if (c->d_hctr) (void)hipFree(c->d_hctr);
   if (c->d_p1cnt) (void)hipFree(c->d_p1cnt);
   if (c->d_bq) (void)hipFree(c->d_bq);
   if (c->d_bcs) (void)hipFree(c->d_bcs);

@@ -66,18 +66,26 @@ struct SymArgs {
   int64_t scr_stride;    // ints per scratch list (>= max degree)
   uint64_t* out;
   // P5, P7, P9, P11, P12: edges with heavy_min < tri[e] <= sym_heavy_nmax()
-  // are left to launch_sym_heavy (0: every edge runs here; needs tri)
+  // go to launch_sym_heavy (0: every edge runs here; needs tri)
   int32_t heavy_min;
-  int32_t heavy_max;     // (launch_sym_heavy) edges with heavy_min < tri[e] <= heavy_max
   int32_t heavy_stream;  // stream N(v2) when d(v2) <= heavy_stream * |S|, else binary searches
+  int32_t nv;            // vertices (the heavy kernel keeps S as a bitmap on small graphs)
+  // the wave-per-edge launch appends each heavy edge to list 0 (|S| <=
+  // heavy_mid) or 1 (hlist + hcap), counters hctr[0..1]; an edge that finds
+  // its list full it counts itself.  The heavy launches pop with hctr[2..3].
+  int32_t* hlist;
+  unsigned long long* hctr;
+  int32_t hcap;
+  int32_t heavy_mid;
 };
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
-// The heavy edges of P5, P7, P9, P11 and P12, one block per edge (a.qctr
-// zero before; edges taken 64 at a time); wide: |S| above sym_heavy_mid().
+// The heavy edges of P5, P7, P9, P11 and P12, one block per edge, popped
+// from list `wide` (|S| above sym_heavy_mid(p)) that launch_sym filled.
 bool sym_heavy_pattern(int p);
 int sym_heavy_nmax();
+int sym_heavy_cap();  // entries per heavy-edge list
 int sym_heavy_stream();
-int sym_heavy_mid();  // |S| up to this: the narrow launch (more blocks per CU)
+int sym_heavy_mid(int p);  // |S| up to this: the narrow launch (more blocks per CU)
 hipError_t launch_sym_heavy(int p, const SymArgs& a, bool wide, int grid, hipStream_t st);
 
 // 4-cycles (P1) by wedges: one wave per root of [rb, re), counters in a

@@ -15,8 +15,13 @@ constexpr int NLIST = 5;      // per-wave lists: S, posS0, posS1, L, posL0
 // block-per-edge kernel for the heavy edges of the q-sum patterns
 constexpr int HV_WAVES = 16;           // waves per block (one edge)
 constexpr int HV_NMAX = 8192;          // largest |S| it takes
-constexpr int HV_NMID = 1024;          // |S| up to this: the narrow shape
 constexpr int HV_WAVES_MID = 8;
+constexpr int HV_BM_BITS = 1 << 18;    // graphs up to this many vertices: S as a bitmap
+// |S| up to this: the narrow shape.  R-MAT 18, narrow width 512 / 1024 /
+// 2048: P9 168 / 184 / 195 ms, P11 217 / 247 / 261, P5 438 / 407 / 424,
+// P7 551 / 500 / 554, P12 511 / 456 / 494.
+template <int P>
+constexpr int heavy_mid() { return P == 9 || P == 11 ? 512 : 1024; }
 // stream N(v2) when d(v2) <= HV_STREAM * |S|, else binary searches (R-MAT 18
 // P9: ratio 2 / 4 / 8 / 16 / 32 / 64 -> 274 / 233 / 227 / 250 / 291 / 357 ms)
 constexpr int HV_STREAM = 8;

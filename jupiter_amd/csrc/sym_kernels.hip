@@ -46,6 +46,14 @@ namespace jk {
 
 using namespace symd;
 
+// the orientation on which edge_value<P> can be non-zero
+template <int P>
+__device__ __forceinline__ bool heavy_orient(int32_t v0, int32_t v1) {
+  if constexpr (P == 5) return v1 < v0;
+  else if constexpr (P == 7) return true;
+  else return v1 > v0;  // P9, P11, P12
+}
+
 template <int P>
 __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
   __shared__ int32_t s_list[WPB][NLIST][CAP];
@@ -65,14 +73,24 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
     const int64_t end = min(a.ne, base + (int64_t)a.chunk);
     for (int64_t e = base; e < end; ++e) {
       const int64_t eid = a.e0 + e;
-      if constexpr (heavy_split<P>()) {
-        if (a.heavy_min > 0) {  // left to sym_heavy_kernel
-          const int32_t t = a.tri[eid];
-          if (t > a.heavy_min && t <= HV_NMAX) continue;
-        }
-      }
       const int32_t v0 = a.src[eid];
       const int32_t v1 = a.ci[eid];
+      if constexpr (heavy_split<P>()) {
+        if (a.heavy_min > 0) {  // heavy: onto a list for sym_heavy_kernel
+          const int32_t t = a.tri[eid];
+          if (t > a.heavy_min && t <= HV_NMAX) {
+            if (!heavy_orient<P>(v0, v1)) continue;  // its value is 0
+            const int wide = t > a.heavy_mid;
+            unsigned long long k = 0;
+            if (ln == 0) k = atomicAdd(a.hctr + wide, 1ull);
+            k = __shfl(k, 0);
+            if (k < (unsigned long long)a.hcap) {
+              if (ln == 0) a.hlist[(int64_t)wide * a.hcap + (int64_t)k] = (int32_t)(eid - a.e0);
+              continue;
+            }
+          }
+        }
+      }
       // lists fit in LDS when the smaller endpoint degree does
       const int dmin = min((int)(a.rp[v0 + 1] - a.rp[v0]), (int)(a.rp[v1 + 1] - a.rp[v1]));
       int32_t* L[NLIST];
@@ -104,12 +122,6 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
 // loads, one LDS probe sequence per element) when d(v2) <= HV_STREAM * |S|,
 // else by binary searches of S's members in N(v2).  The per-edge value is the
 // one edge_value<P> returns (the same formulas on the same q, b and c).
-template <int P>
-__device__ __forceinline__ bool heavy_orient(int32_t v0, int32_t v1) {
-  if constexpr (P == 5) return v1 < v0;
-  else if constexpr (P == 7) return true;
-  else return v1 > v0;  // P9, P11, P12
-}
 
 __device__ __forceinline__ uint32_t hv_hash(int32_t x, int lg) { return ((uint32_t)x * 2654435761u) >> (32 - lg); }
 
@@ -123,44 +135,44 @@ __device__ __forceinline__ int hv_has(const int32_t* tab, int mask, int lg, int3
   }
 }
 
-// Two shapes: |S| <= NMAX of 1024 with 8 waves (small LDS, several blocks per
-// CU) and of 8192 with 16 waves (one block per CU); the launch takes the
-// edges with heavy_min < |S| <= heavy_max.
+// Two shapes: |S| <= NMAX of 512 or 1024 with 8 waves (small LDS, several
+// blocks per CU) and of 8192 with 16 waves (one block per CU), each popping
+// the edges of its list one at a time.
 template <int P, int NMAX, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
   constexpr int TMAX = NMAX <= 1024 ? 4 * NMAX : 2 * NMAX;  // hash slots
+  constexpr int BMW = HV_BM_BITS / 32;
+  // S as a set: a bitmap over the vertex ids when the graph has at most
+  // HV_BM_BITS vertices (one LDS read per test), else the hash table
+  __shared__ int32_t s_tab[TMAX > BMW ? TMAX : BMW];
   __shared__ int32_t s_S[NMAX];
-  __shared__ int32_t s_tab[TMAX];  // vertex ids of S, -1 = empty
   __shared__ int64_t s_red[WAVES];
-  __shared__ int64_t s_base;
-  __shared__ uint64_t s_mask;
+  __shared__ int64_t s_k;
   __shared__ int32_t s_n, s_next;
   constexpr int NT = 64 * WAVES;
   const int tid = threadIdx.x;
   const int wv = tid >> 6;
   const int ln = lane();
-  for (int k = tid; k < TMAX; k += NT) s_tab[k] = -1;
+  const bool bm = a.nv <= HV_BM_BITS;
+  if (bm) {
+    for (int k = tid; k < BMW; k += NT) s_tab[k] = 0;
+  } else {
+    for (int k = tid; k < TMAX; k += NT) s_tab[k] = -1;
+  }
+  // the list this shape takes: 0 (|S| <= heavy_mid) or 1
+  const int wide = NMAX > a.heavy_mid;
+  const int32_t* list = a.hlist + (int64_t)wide * a.hcap;
+  const int64_t cnt = min((int64_t)a.hctr[wide], (int64_t)a.hcap);
   uint64_t acc = 0;  // thread 0's
   for (;;) {
-    if (tid == 0) s_base = (int64_t)atomicAdd(a.qctr, 64ull);
+    if (tid == 0) s_k = (int64_t)atomicAdd(a.hctr + 2 + wide, 1ull);
     __syncthreads();
-    const int64_t base = s_base;
-    if (base >= a.ne) break;
-    if (wv == 0) {
-      bool h = false;
-      if (base + ln < a.ne) {
-        const int64_t eid = a.e0 + base + ln;
-        const int32_t t = a.tri[eid];
-        h = t > a.heavy_min && t <= a.heavy_max && t <= NMAX && heavy_orient<P>(a.src[eid], a.ci[eid]);
-      }
-      const uint64_t m = __ballot(h);
-      if (ln == 0) s_mask = m;
-    }
-    __syncthreads();
-    for (uint64_t m = s_mask; m; m &= m - 1) {
-      const int64_t eid = a.e0 + base + __builtin_ctzll(m);
+    const int64_t k = s_k;
+    if (k >= cnt) break;
+    const int64_t eid = a.e0 + list[k];
+    const int n = a.tri[eid];
+    if (n <= NMAX) {  // always: the launch listed it by this bound
       const int32_t v0 = a.src[eid], v1 = a.ci[eid];
-      const int n = a.tri[eid];
       const int64_t b0 = a.rp[v0], b1 = a.rp[v1];
       const int d0 = (int)(a.rp[v0 + 1] - b0), d1 = (int)(a.rp[v1 + 1] - b1);
       const int32_t* N0 = a.ci + b0;
@@ -196,18 +208,23 @@ __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
         if (f && k < n) {  // k < n always: the triangle array holds |S|
           s_S[k] = x;
-          uint32_t h = hv_hash(x, lg);
-          while (atomicCAS(&s_tab[h], -1, x) != -1) h = (h + 1) & (uint32_t)mask;
+          if (bm) {
+            atomicOr(reinterpret_cast<uint32_t*>(&s_tab[x >> 5]), 1u << (x & 31));
+          } else {
+            uint32_t h = hv_hash(x, lg);
+            while (atomicCAS(&s_tab[h], -1, x) != -1) h = (h + 1) & (uint32_t)mask;
+          }
         }
       }
       __syncthreads();
+      const int nS = min(n, s_n);  // = n: the members written to s_S
       // q(v2) for the members, one wave per member at a time
       int64_t part = 0;  // lane 0's
       for (;;) {
         int j = 0;
         if (ln == 0) j = atomicAdd(&s_next, 1);
         j = __shfl(j, 0);
-        if (j >= n) break;
+        if (j >= nS) break;
         const int32_t v2 = s_S[j];
         if constexpr (P == 7 || P == 9 || P == 11) {
           if (v2 <= v1) continue;
@@ -225,14 +242,27 @@ __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
         int q = 0;
         if (d2 <= a.heavy_stream * n) {
           int k = ln;
-          for (; k + 192 < d2; k += 256) {  // four loads in flight
-            const int32_t x0 = N2[k], x1 = N2[k + 64], x2 = N2[k + 128], x3 = N2[k + 192];
-            q += hv_has(s_tab, mask, lg, x0) + hv_has(s_tab, mask, lg, x1) + hv_has(s_tab, mask, lg, x2) +
-                 hv_has(s_tab, mask, lg, x3);
+          if (bm) {
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(s_tab);
+            for (; k + 192 < d2; k += 256) {  // four loads in flight
+              const int32_t x0 = N2[k], x1 = N2[k + 64], x2 = N2[k + 128], x3 = N2[k + 192];
+              q += ((w[x0 >> 5] >> (x0 & 31)) & 1) + ((w[x1 >> 5] >> (x1 & 31)) & 1) +
+                   ((w[x2 >> 5] >> (x2 & 31)) & 1) + ((w[x3 >> 5] >> (x3 & 31)) & 1);
+            }
+            for (; k < d2; k += 64) {
+              const int32_t x = N2[k];
+              q += (w[x >> 5] >> (x & 31)) & 1;
+            }
+          } else {
+            for (; k + 192 < d2; k += 256) {
+              const int32_t x0 = N2[k], x1 = N2[k + 64], x2 = N2[k + 128], x3 = N2[k + 192];
+              q += hv_has(s_tab, mask, lg, x0) + hv_has(s_tab, mask, lg, x1) + hv_has(s_tab, mask, lg, x2) +
+                   hv_has(s_tab, mask, lg, x3);
+            }
+            for (; k < d2; k += 64) q += hv_has(s_tab, mask, lg, N2[k]);
           }
-          for (; k < d2; k += 64) q += hv_has(s_tab, mask, lg, N2[k]);
         } else {
-          for (int k = ln; k < n; k += 64) q += in_sorted(N2, d2, s_S[k]);  // (P5: members > v2 only)
+          for (int k = ln; k < nS; k += 64) q += in_sorted(N2, d2, s_S[k]);  // (P5: members > v2 only)
         }
         q = wsum(q);
         // t(v0, v2) and t(v1, v2) from v2's positions in N(v0) and N(v1)
@@ -261,8 +291,14 @@ __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
         else if constexpr (P == 12) c = n >= 3 ? (int64_t)(n - 3) * c : 0;
         acc += (uint64_t)c;
       }
-      for (int k = tid; k <= mask; k += NT) s_tab[k] = -1;
+      if (bm) {
+        for (int k = tid; k < nS; k += NT) s_tab[s_S[k] >> 5] = 0;
+      } else {
+        for (int k = tid; k <= mask; k += NT) s_tab[k] = -1;
+      }
       __syncthreads();
+    } else {
+      __syncthreads();  // s_k read by all before it is taken again
     }
   }
   if (tid == 0 && acc) atomicAdd((unsigned long long*)a.out, (unsigned long long)acc);
@@ -663,15 +699,16 @@ hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st) {
 template <int P>
 static hipError_t launch_heavy_p(const SymArgs& a, bool wide, int grid, hipStream_t st) {
   if (wide) hipLaunchKernelGGL((sym_heavy_kernel<P, HV_NMAX, HV_WAVES>), dim3(grid), dim3(64 * HV_WAVES), 0, st, a);
-  else hipLaunchKernelGGL((sym_heavy_kernel<P, HV_NMID, HV_WAVES_MID>), dim3(grid), dim3(64 * HV_WAVES_MID), 0, st, a);
+  else hipLaunchKernelGGL((sym_heavy_kernel<P, heavy_mid<P>(), HV_WAVES_MID>), dim3(grid), dim3(64 * HV_WAVES_MID), 0, st, a);
   return hipGetLastError();
 }
 
 bool sym_heavy_pattern(int p) { return p == 5 || p == 7 || p == 9 || p == 11 || p == 12; }
 int sym_heavy_nmax() { return HV_NMAX; }
 int sym_heavy_stream() { return HV_STREAM; }
+int sym_heavy_cap() { return 1 << 22; }
 
-int sym_heavy_mid() { return HV_NMID; }
+int sym_heavy_mid(int p) { return p == 9 || p == 11 ? heavy_mid<9>() : heavy_mid<5>(); }
 
 hipError_t launch_sym_heavy(int p, const SymArgs& a, bool wide, int grid, hipStream_t st) {
   switch (p) {

@@ -232,11 +232,13 @@ def test_heavy_edge_kernel_thresholds(ctx):
         G.free()
 
 
-def test_heavy_edge_kernel_wide_shape(ctx):
+@pytest.mark.parametrize("nv_pad", [0, 300_000])
+def test_heavy_edge_kernel_wide_shape(ctx, nv_pad):
     """An edge whose endpoints share 1,200 neighbours (two hubs over a sparse
     random graph) runs in the 8192-wide shape of the heavy kernel, the
-    others in the 1024-wide one or the wave-per-edge kernel: equal to the
-    oracle at every threshold."""
+    others in the narrow one or the wave-per-edge kernel: equal to the
+    oracle at every threshold.  Padded with isolated vertices beyond 2^18,
+    the kernel keeps S in its hash table instead of a vertex bitmap."""
     rng = np.random.default_rng(3)
     m = 1200
     com = np.arange(2, m + 2)
@@ -244,7 +246,7 @@ def test_heavy_edge_kernel_wide_shape(ctx):
     keep = rng.random(len(iu)) < 0.01
     src = np.concatenate([[0], np.zeros(m, np.int64), np.ones(m, np.int64), iu[keep] + 2])
     dst = np.concatenate([[1], com, com, ju[keep] + 2])
-    g = J.CSR.from_edges(m + 2, src, dst)
+    g = J.CSR.from_edges(max(m + 2, nv_pad), src, dst)
     G = ctx.upload(g)
     want = {p: O.count(g.rowptr, g.col, J.PATTERNS[p]) for p in HEAVY}
     want2 = {p: O.count(g.rowptr, g.col, J.PATTERNS[p], 0, 2) for p in HEAVY}  # the two hubs
@@ -252,7 +254,7 @@ def test_heavy_edge_kernel_wide_shape(ctx):
         for h in [1, 32, 1000, 8192]:
             ctx.set_tuning(sym_heavy_min=h)
             for p in HEAVY:
-                _say(f"two hubs heavy_min {h} {p}")
+                _say(f"two hubs nv {g.nv} heavy_min {h} {p}")
                 assert G.count(p)[0] == want[p], (h, p)
                 assert G.count(p, (0, 2))[0] == want2[p], (h, p)
     finally:
