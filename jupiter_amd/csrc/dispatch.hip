@@ -147,16 +147,26 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   // reset per query by ctx_reset_counters before it is used here.
   int rc = ensure_sym_aux(g, query_needs_tri(c, p), p == 3, st);
   if (rc) return rc;
-  // the heavy-edge split needs its two edge lists (allocated once per
-  // context, so not while a launch sequence is being captured) and edge
-  // indices of a range that fit 32 bits
-  bool heavy = heavy_split_on(c, p) && g->sym_rp_h[re] - g->sym_rp_h[rb] < INT32_MAX;
-  if (heavy && !c->d_hlist) {
+  // the heavy-edge split needs its three edge lists, each as long as the
+  // range's edge count up to 2^26 (grown per context, so not while a launch
+  // sequence is being captured; edges beyond a full list run in the
+  // wave-per-edge launch), and edge indices of a range that fit 32 bits
+  const int64_t ne_r = g->sym_rp_h[re] - g->sym_rp_h[rb];
+  bool heavy = heavy_split_on(c, p) && ne_r < INT32_MAX;
+  const int64_t hwant = std::min<int64_t>(std::max<int64_t>(ne_r, 1), (int64_t)1 << 26);
+  if (heavy && c->hcap < hwant) {
     if (c->capturing) {
-      heavy = false;
+      heavy = c->d_hlist != nullptr;  // the lists it has
     } else {
-      if (hipMalloc(&c->d_hlist, sizeof(int32_t) * 2 * (size_t)jk::sym_heavy_cap()) != hipSuccess) return JGPM_E_NOMEM;
-      if (hipMalloc(&c->d_hctr, sizeof(unsigned long long) * 4) != hipSuccess) return JGPM_E_NOMEM;
+      if (c->d_hlist) {
+        if (hipStreamSynchronize(st) != hipSuccess) return JGPM_E_HIP;
+        (void)hipFree(c->d_hlist);
+        c->d_hlist = nullptr;
+        c->hcap = 0;
+      }
+      if (hipMalloc(&c->d_hlist, sizeof(int32_t) * 3 * (size_t)hwant) != hipSuccess) return JGPM_E_NOMEM;
+      c->hcap = hwant;
+      if (!c->d_hctr && hipMalloc(&c->d_hctr, sizeof(unsigned long long) * 6) != hipSuccess) return JGPM_E_NOMEM;
     }
   }
   const int wpb = jk::sym_waves_per_block();
@@ -192,10 +202,10 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   a.nv = g->sym.nv;
   a.hlist = c->d_hlist;
   a.hctr = c->d_hctr;
-  a.hcap = jk::sym_heavy_cap();
+  a.hcap = (int32_t)c->hcap;
   a.heavy_mid = heavy ? jk::sym_heavy_mid(p) : 0;
   // list counters to zero (a kernel, so the sequence can be captured)
-  if (heavy && zero_u64(reinterpret_cast<uint64_t*>(c->d_hctr), 4, st)) return JGPM_E_HIP;
+  if (heavy && zero_u64(reinterpret_cast<uint64_t*>(c->d_hctr), 6, st)) return JGPM_E_HIP;
   static const char* const kSymName[13] = {"", "sym_p1", "sym_p2", "sym_p3", "sym_p4", "sym_p5", "sym_p6",
                                            "sym_p7", "sym_p8", "sym_p9", "sym_p10", "sym_p11", "sym_p12"};
   prof_begin(c, st, kSymName[p]);
@@ -203,14 +213,16 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   prof_end(c, st);
   if (e != hipSuccess) return JGPM_E_HIP;
   if (heavy) {
-    // the edges the wave-per-edge launch listed, in two shapes: |S| up to
+    // the edges the wave-per-edge launch listed, in three shapes: |S| up to
     // sym_heavy_mid(p) with a small LDS table and several blocks per CU,
-    // wider ones one block per CU
+    // up to sym_heavy_nmax() one block per CU, wider ones (graphs of at
+    // most 2^20 vertices) with S in the list scratch, one slice per block
     static const char* const kHeavyName[13] = {"", "", "", "", "", "sym_p5_heavy", "", "sym_p7_heavy", "",
                                                "sym_p9_heavy", "", "sym_p11_heavy", "sym_p12_heavy"};
-    for (int wide = 0; wide < 2 && e == hipSuccess; ++wide) {
+    const int shapes = g->sym.nv <= (1 << 20) ? 3 : 2;
+    for (int shape = 0; shape < shapes && e == hipSuccess; ++shape) {
       prof_begin(c, st, kHeavyName[p]);
-      e = jk::launch_sym_heavy(p, a, wide != 0, wide ? c->num_cu : 4 * c->num_cu, st);
+      e = jk::launch_sym_heavy(p, a, shape, shape ? c->num_cu : 4 * c->num_cu, st);
       prof_end(c, st);
     }
   }

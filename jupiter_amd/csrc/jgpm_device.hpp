@@ -33,8 +33,9 @@ struct jgpm_ctx_s {
   unsigned long long* d_q64 = nullptr;  // CTX_SLOTS 64-bit work-queue counters
   int32_t* d_sscr = nullptr;  // list scratch of the symmetric-pattern kernels
   size_t sscr_ints = 0;
-  int32_t* d_hlist = nullptr;  // heavy-edge lists of P5/P7/P9/P11/P12 (2 x sym_heavy_cap() ints)
-  unsigned long long* d_hctr = nullptr;  // their append and pop counters (4)
+  int32_t* d_hlist = nullptr;  // heavy-edge lists of P5/P7/P9/P11/P12 (3 x hcap ints)
+  int64_t hcap = 0;
+  unsigned long long* d_hctr = nullptr;  // their append and pop counters (6)
   int32_t* d_p1cnt = nullptr;  // wedge counters of the 4-cycle kernel (all zero between launches)
   size_t p1cnt_ints = 0;
   // batched / served launches: query table, chunk prefix, per-query counts

@@ -71,8 +71,10 @@ struct SymArgs {
   int32_t heavy_stream;  // stream N(v2) when d(v2) <= heavy_stream * |S|, else binary searches
   int32_t nv;            // vertices (the heavy kernel keeps S as a bitmap on small graphs)
   // the wave-per-edge launch appends each heavy edge to list 0 (|S| <=
-  // heavy_mid) or 1 (hlist + hcap), counters hctr[0..1]; an edge that finds
-  // its list full it counts itself.  The heavy launches pop with hctr[2..3].
+  // heavy_mid), 1 (|S| <= sym_heavy_nmax()) or 2 (beyond, on graphs of at
+  // most 2^20 vertices), at hlist + list * hcap, counters hctr[0..2]; an edge
+  // that finds its list full it counts itself.  The heavy launches pop with
+  // hctr[3..5].
   int32_t* hlist;
   unsigned long long* hctr;
   int32_t hcap;
@@ -80,13 +82,13 @@ struct SymArgs {
 };
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
 // The heavy edges of P5, P7, P9, P11 and P12, one block per edge, popped
-// from list `wide` (|S| above sym_heavy_mid(p)) that launch_sym filled.
+// from list `shape` (0, 1, 2 as above) that launch_sym filled; shape 2 keeps
+// S in the block's slice of a.scr (a.scr_stride ints, grid slices).
 bool sym_heavy_pattern(int p);
 int sym_heavy_nmax();
-int sym_heavy_cap();  // entries per heavy-edge list
 int sym_heavy_stream();
 int sym_heavy_mid(int p);  // |S| up to this: the narrow launch (more blocks per CU)
-hipError_t launch_sym_heavy(int p, const SymArgs& a, bool wide, int grid, hipStream_t st);
+hipError_t launch_sym_heavy(int p, const SymArgs& a, int shape, int grid, hipStream_t st);
 
 // 4-cycles (P1) by wedges: one wave per root of [rb, re), counters in a
 // per-wave dense array of nv ints (cnt + wave * nv), zero before and after.

@@ -125,6 +125,8 @@ static int64_t footprint(jgpm_graph_s* g, int p, const jgpm_ctx_s* c, int grid) 
   if (!is_dag_pattern(p)) {
     if (!g->d_src) b += 4 * g->sym.nnz;
     if (!g->d_tri && jgpm::query_needs_tri(c, p)) b += 4 * g->sym.nnz;
+    if (jk::sym_heavy_pattern(p))  // the heavy-edge lists of the context that runs it
+      b += 12 * std::max<int64_t>(0, std::min<int64_t>(g->sym.nnz, (int64_t)1 << 26) - c->hcap);
     if (!g->d_nsum && p == JGPM_P3) b += 8 * (int64_t)g->sym.nv;
     b += (int64_t)grid * 4 * 5 * 4 * std::max(1, g->sym_mmin);  // list scratch of the batch kernel
   } else if (g->dag.max_deg > jk::tier_max_deg(jk::TIER_LDS16)) {

@@ -17,6 +17,7 @@ constexpr int HV_WAVES = 16;           // waves per block (one edge)
 constexpr int HV_NMAX = 8192;          // largest |S| it takes
 constexpr int HV_WAVES_MID = 8;
 constexpr int HV_BM_BITS = 1 << 18;    // graphs up to this many vertices: S as a bitmap
+constexpr int HV_BM_HUGE = 1 << 20;    // ... and |S| > HV_NMAX runs in the huge shape
 // |S| up to this: the narrow shape.  R-MAT 18, narrow width 512 / 1024 /
 // 2048: P9 168 / 184 / 195 ms, P11 217 / 247 / 261, P5 438 / 407 / 424,
 // P7 551 / 500 / 554, P12 511 / 456 / 494.

@@ -78,9 +78,9 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
       if constexpr (heavy_split<P>()) {
         if (a.heavy_min > 0) {  // heavy: onto a list for sym_heavy_kernel
           const int32_t t = a.tri[eid];
-          if (t > a.heavy_min && t <= HV_NMAX) {
+          if (t > a.heavy_min && (t <= HV_NMAX || a.nv <= HV_BM_HUGE)) {
             if (!heavy_orient<P>(v0, v1)) continue;  // its value is 0
-            const int wide = t > a.heavy_mid;
+            const int wide = t > HV_NMAX ? 2 : t > a.heavy_mid;
             unsigned long long k = 0;
             if (ln == 0) k = atomicAdd(a.hctr + wide, 1ull);
             k = __shfl(k, 0);
@@ -135,17 +135,21 @@ __device__ __forceinline__ int hv_has(const int32_t* tab, int mask, int lg, int3
   }
 }
 
-// Two shapes: |S| <= NMAX of 512 or 1024 with 8 waves (small LDS, several
-// blocks per CU) and of 8192 with 16 waves (one block per CU), each popping
-// the edges of its list one at a time.
+// Three shapes, each popping the edges of its list one at a time: |S| <=
+// NMAX of 512 or 1024 with 8 waves (small LDS, several blocks per CU), of
+// 8192 with 16 waves (one block per CU), and (NMAX = 0) any |S| on graphs
+// of at most HV_BM_HUGE vertices, S then in the block's slice of the list
+// scratch and the set a bitmap of up to 128 KB.
 template <int P, int NMAX, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
-  constexpr int TMAX = NMAX <= 1024 ? 4 * NMAX : 2 * NMAX;  // hash slots
-  constexpr int BMW = HV_BM_BITS / 32;
+  constexpr bool HUGE_SHAPE = NMAX == 0;
+  constexpr int TMAX = HUGE_SHAPE ? 0 : NMAX <= 1024 ? 4 * NMAX : 2 * NMAX;  // hash slots
+  constexpr int BMW = (HUGE_SHAPE ? HV_BM_HUGE : HV_BM_BITS) / 32;
   // S as a set: a bitmap over the vertex ids when the graph has at most
   // HV_BM_BITS vertices (one LDS read per test), else the hash table
   __shared__ int32_t s_tab[TMAX > BMW ? TMAX : BMW];
-  __shared__ int32_t s_S[NMAX];
+  __shared__ int32_t s_Sl[HUGE_SHAPE ? 1 : NMAX];
+  int32_t* const s_S = HUGE_SHAPE ? a.scr + (int64_t)blockIdx.x * a.scr_stride : s_Sl;
   __shared__ int64_t s_red[WAVES];
   __shared__ int64_t s_k;
   __shared__ int32_t s_n, s_next;
@@ -153,25 +157,25 @@ __global__ void __launch_bounds__(64 * WAVES) sym_heavy_kernel(SymArgs a) {
   const int tid = threadIdx.x;
   const int wv = tid >> 6;
   const int ln = lane();
-  const bool bm = a.nv <= HV_BM_BITS;
+  const bool bm = HUGE_SHAPE || a.nv <= HV_BM_BITS;
   if (bm) {
     for (int k = tid; k < BMW; k += NT) s_tab[k] = 0;
   } else {
     for (int k = tid; k < TMAX; k += NT) s_tab[k] = -1;
   }
-  // the list this shape takes: 0 (|S| <= heavy_mid) or 1
-  const int wide = NMAX > a.heavy_mid;
+  // the list this shape takes: 0 (|S| <= heavy_mid), 1 or 2 (huge)
+  const int wide = HUGE_SHAPE ? 2 : NMAX > a.heavy_mid;
   const int32_t* list = a.hlist + (int64_t)wide * a.hcap;
   const int64_t cnt = min((int64_t)a.hctr[wide], (int64_t)a.hcap);
   uint64_t acc = 0;  // thread 0's
   for (;;) {
-    if (tid == 0) s_k = (int64_t)atomicAdd(a.hctr + 2 + wide, 1ull);
+    if (tid == 0) s_k = (int64_t)atomicAdd(a.hctr + 3 + wide, 1ull);
     __syncthreads();
     const int64_t k = s_k;
     if (k >= cnt) break;
     const int64_t eid = a.e0 + list[k];
     const int n = a.tri[eid];
-    if (n <= NMAX) {  // always: the launch listed it by this bound
+    if (HUGE_SHAPE ? n <= a.scr_stride && a.nv <= HV_BM_HUGE : n <= NMAX) {  // always: listed by these bounds
       const int32_t v0 = a.src[eid], v1 = a.ci[eid];
       const int64_t b0 = a.rp[v0], b1 = a.rp[v1];
       const int d0 = (int)(a.rp[v0 + 1] - b0), d1 = (int)(a.rp[v1 + 1] - b1);
@@ -697,8 +701,9 @@ hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st) {
 }
 
 template <int P>
-static hipError_t launch_heavy_p(const SymArgs& a, bool wide, int grid, hipStream_t st) {
-  if (wide) hipLaunchKernelGGL((sym_heavy_kernel<P, HV_NMAX, HV_WAVES>), dim3(grid), dim3(64 * HV_WAVES), 0, st, a);
+static hipError_t launch_heavy_p(const SymArgs& a, int shape, int grid, hipStream_t st) {
+  if (shape == 2) hipLaunchKernelGGL((sym_heavy_kernel<P, 0, HV_WAVES>), dim3(grid), dim3(64 * HV_WAVES), 0, st, a);
+  else if (shape == 1) hipLaunchKernelGGL((sym_heavy_kernel<P, HV_NMAX, HV_WAVES>), dim3(grid), dim3(64 * HV_WAVES), 0, st, a);
   else hipLaunchKernelGGL((sym_heavy_kernel<P, heavy_mid<P>(), HV_WAVES_MID>), dim3(grid), dim3(64 * HV_WAVES_MID), 0, st, a);
   return hipGetLastError();
 }
@@ -706,17 +711,16 @@ static hipError_t launch_heavy_p(const SymArgs& a, bool wide, int grid, hipStrea
 bool sym_heavy_pattern(int p) { return p == 5 || p == 7 || p == 9 || p == 11 || p == 12; }
 int sym_heavy_nmax() { return HV_NMAX; }
 int sym_heavy_stream() { return HV_STREAM; }
-int sym_heavy_cap() { return 1 << 22; }
 
 int sym_heavy_mid(int p) { return p == 9 || p == 11 ? heavy_mid<9>() : heavy_mid<5>(); }
 
-hipError_t launch_sym_heavy(int p, const SymArgs& a, bool wide, int grid, hipStream_t st) {
+hipError_t launch_sym_heavy(int p, const SymArgs& a, int shape, int grid, hipStream_t st) {
   switch (p) {
-    case 5: return launch_heavy_p<5>(a, wide, grid, st);
-    case 7: return launch_heavy_p<7>(a, wide, grid, st);
-    case 9: return launch_heavy_p<9>(a, wide, grid, st);
-    case 11: return launch_heavy_p<11>(a, wide, grid, st);
-    case 12: return launch_heavy_p<12>(a, wide, grid, st);
+    case 5: return launch_heavy_p<5>(a, shape, grid, st);
+    case 7: return launch_heavy_p<7>(a, shape, grid, st);
+    case 9: return launch_heavy_p<9>(a, shape, grid, st);
+    case 11: return launch_heavy_p<11>(a, shape, grid, st);
+    case 12: return launch_heavy_p<12>(a, shape, grid, st);
   }
   return hipErrorInvalidValue;
 }

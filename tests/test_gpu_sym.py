@@ -262,6 +262,42 @@ def test_heavy_edge_kernel_wide_shape(ctx, nv_pad):
     G.free()
 
 
+@pytest.mark.parametrize("nv_pad", [0, 1_100_000])
+def test_heavy_edge_kernel_huge_shape(ctx, nv_pad):
+    """Two hubs sharing 9,000 neighbours: the hub edge's S exceeds the LDS
+    list of the 8192-wide shape, so on a graph of at most 2^20 vertices it
+    runs in the huge shape (S in the list scratch, a 128 KB vertex bitmap),
+    beyond 2^20 vertices in the wave-per-edge kernel.  P5, P7 and P9 equal
+    the oracle; P11 and P12 (a minute each in the oracle) equal the
+    wave-per-edge kernel's counts, which the other tests hold to the oracle."""
+    rng = np.random.default_rng(4)
+    m = 9000
+    com = np.arange(2, m + 2)
+    a = rng.integers(0, m, 40000)
+    b = rng.integers(0, m, 40000)
+    keep = a != b
+    src = np.concatenate([[0], np.zeros(m, np.int64), np.ones(m, np.int64), a[keep] + 2])
+    dst = np.concatenate([[1], com, com, b[keep] + 2])
+    g = J.CSR.from_edges(max(m + 2, nv_pad), src, dst)
+    G = ctx.upload(g)
+    want = {}
+    for p in HEAVY:
+        if p in ("P5", "P7", "P9"):
+            want[p] = (O.count(g.rowptr, g.col, J.PATTERNS[p]), O.count(g.rowptr, g.col, J.PATTERNS[p], 0, 2))
+        else:
+            ctx.set_tuning(sym_heavy_min=8192)
+            want[p] = (G.count(p)[0], G.count(p, (0, 2))[0])
+    try:
+        for h in [1, 32, 8192]:
+            ctx.set_tuning(sym_heavy_min=h)
+            for p in HEAVY:
+                _say(f"9000 common nv {g.nv} heavy_min {h} {p}")
+                assert (G.count(p)[0], G.count(p, (0, 2))[0]) == want[p], (h, p)
+    finally:
+        ctx.set_tuning(sym_heavy_min=-1)
+    G.free()
+
+
 def test_heavy_edge_kernel_rmat16(ctx):
     """On R-MAT 16 (edges with up to ~1,000 common neighbours) the split run
     at the default threshold equals the wave-per-edge run over the whole
