@@ -758,9 +758,11 @@ struct Lay {
 template <int G, int WMAX, int K, bool BYTES, bool LROWS, int PHASE = 0>
 __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                              const int32_t* __restrict__ ord, int64_t b0, int d, int ratio, unsigned char* ar,
-                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc) {
+                             uint64_t* rowp, int32_t* s_wtot, uint64_t& acc, uint32_t* grow = nullptr) {
   // PHASE 0: build and count; 1: build only (rows left in rowp); 2: count
-  // only (rowp already holds the rows of this root)
+  // only (rowp already holds the rows of this root).  grow: a global row
+  // counter (zero before) that several groups counting the same root share;
+  // null: the group's own counter in LDS
   static_assert(PHASE == 0 || (!BYTES && K >= 4), "split phases are for counting with rows");
   // Exactness of the fp4 matrix-core counts (tri_mfma1, tri_mfma, tri4_mfma):
   // every product of two set e2m1 elements is exactly 1, an accumulator entry
@@ -986,13 +988,17 @@ __device__ void process_root(const int64_t* __restrict__ rp, const int32_t* __re
       if constexpr (!DYN) {
         return cur + G;
       } else {
-        // the counter's LDS address from a zero made here (the empty asm
-        // keeps it from being hoisted): a loop-invariant address register
-        // would be spilled across the row's matrix-core work and reloaded
-        int z = 0;
-        asm volatile("" : "+v"(z));
         int t = 0;
-        if (ln == 0) t = atomicAdd(s_wtot + G + z, 1);
+        if (grow) {
+          if (ln == 0) t = (int)atomicAdd(grow, 1u);
+        } else {
+          // the counter's LDS address from a zero made here (the empty asm
+          // keeps it from being hoisted): a loop-invariant address register
+          // would be spilled across the row's matrix-core work and reloaded
+          int z = 0;
+          asm volatile("" : "+v"(z));
+          if (ln == 0) t = atomicAdd(s_wtot + G + z, 1);
+        }
         return __builtin_amdgcn_readfirstlane(t);
       }
     };

@@ -187,6 +187,95 @@ hipError_t launch_clique_split(int k, bool build, const CliqueArgs& a, int grid,
 size_t split_slot_words() { return ROWW; }
 
 // ---------------------------------------------------------------------------
+// Split form of the 2048-wide tier for 5- to 7-cliques.  The fused kernel
+// counts a root with one block of 8 waves (two per SIMD, the registers of
+// the 2x2 matrix-core tiles), so the roots of a query run about one per CU
+// and the densest ones leave the device waiting.  Here a build kernel with
+// 16 waves per root leaves each root's rows (512 KiB) in a global slot, and
+// the count kernel lets a.rep blocks take the rows of one root from a shared
+// global row counter.  (4-cliques keep the fused kernel: their whole-root
+// product is not split by rows.)
+constexpr int GLB_WMAX = 32;
+constexpr size_t GSLOT = (size_t)64 * GLB_WMAX * GLB_WMAX;
+
+template <int K>
+__global__ void __launch_bounds__(64 * 16) clique_glb_build_kernel(CliqueArgs a) {
+  constexpr int G = 16;
+  using LY = Lay<G, GLB_WMAX, K, false, false, false>;
+  __shared__ __attribute__((aligned(16))) unsigned char s_arena[LY::SLOT];
+  __shared__ int32_t s_wtot[G + 1];
+  __shared__ int64_t s_task;
+  uint64_t acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = (int64_t)atomicAdd(a.qctr, 1u);
+    __syncthreads();
+    const int64_t task = s_task;
+    __syncthreads();
+    if (task >= a.ntasks) break;
+    const int32_t v0 = a.ids[task];
+    const int64_t b0 = a.rp[v0];
+    const int d = (int)(a.rp[v0 + 1] - b0);
+    if (d < a.dmin || d > a.dmax || d > LY::DM) continue;
+    process_root<G, GLB_WMAX, K, false, false, 1>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena,
+                                                  a.gscr + (size_t)task * GSLOT, s_wtot, acc);
+  }
+}
+
+template <int K>
+__global__ void __launch_bounds__(64 * 8) clique_glb_count_kernel(CliqueArgs a) {
+  constexpr int G = 8;
+  using LY = Lay<G, GLB_WMAX, K, false, false, true, false>;
+  __shared__ __attribute__((aligned(16))) unsigned char s_arena[LY::SLOT];
+  __shared__ int32_t s_wtot[G + 1];
+  __shared__ int64_t s_item;
+  uint64_t acc = 0;
+  // Workgroups are dealt to the 8 XCDs round robin, so block b runs on XCD
+  // b % 8.  Queue q (counter a.rowctr[q - 8]) holds the roots q, q + 8, ...,
+  // each as rep items that start together: the blocks sharing a root's
+  // 512 KiB of rows are on one XCD and read them through its L2.  A block
+  // drains its own XCD's queue, then helps the others.
+  uint32_t* const qx = a.rowctr - 8;
+  const int xcd = (int)(blockIdx.x & 7);
+  for (int s = 0; s < 8; ++s) {
+    const int q = (xcd + s) & 7;
+    const int64_t nq = a.ntasks > q ? ((a.ntasks - 1 - q) / 8 + 1) * a.rep : 0;
+    for (;;) {
+      if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(qx + q, 1u);
+      __syncthreads();
+      const int64_t item = s_item;
+      __syncthreads();
+      if (item >= nq) break;
+      const int64_t task = (item / a.rep) * 8 + q;
+      const int32_t v0 = a.ids[task];
+      const int64_t b0 = a.rp[v0];
+      const int d = (int)(a.rp[v0 + 1] - b0);
+      if (d < a.dmin || d > a.dmax || d > LY::DM) continue;
+      process_root<G, GLB_WMAX, K, false, false, 2>(a.rp, a.ci, a.ord, b0, d, a.ratio, s_arena,
+                                                    a.gscr + (size_t)task * GSLOT, s_wtot, acc, a.rowctr + task);
+    }
+  }
+  block_add<64 * G>(acc, a.out);
+}
+
+template <int K>
+static hipError_t launch_glb_split_k(bool build, const CliqueArgs& a, int grid, hipStream_t st) {
+  if (build) hipLaunchKernelGGL(clique_glb_build_kernel<K>, dim3(grid), dim3(64 * 16), 0, st, a);
+  else hipLaunchKernelGGL(clique_glb_count_kernel<K>, dim3(grid), dim3(64 * 8), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_clique_glb_split(int k, bool build, const CliqueArgs& a, int grid, hipStream_t st) {
+  switch (k) {
+    case 5: return launch_glb_split_k<5>(build, a, grid, st);
+    case 6: return launch_glb_split_k<6>(build, a, grid, st);
+    case 7: return launch_glb_split_k<7>(build, a, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+size_t glb_slot_words() { return GSLOT; }
+
+// ---------------------------------------------------------------------------
 // Host-side launch table.
 template <int G, int WPB, int WMAX, int K, bool BYTES, bool LROWS>
 static hipError_t launch_one(const CliqueArgs& a, int grid, hipStream_t st) {

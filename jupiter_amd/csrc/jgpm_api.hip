@@ -563,6 +563,44 @@ int enqueue_clique(jgpm_graph_s* g, int k, bool bytes, int64_t rb, int64_t re, i
         }
         continue;
       }
+      if (tier == jk::TIER_GLB32 && !bytes && k >= 5 && !gscr && !c->capturing) {
+        // split form: build with 16 waves per root into 512 KiB row slots,
+        // then count with several blocks per root sharing its rows through a
+        // per-root row counter; chunks of roots bound the slots to 1 GiB
+        const size_t slot = jk::glb_slot_words();
+        const int64_t per = std::max<int64_t>(1, ((int64_t)1 << 27) / (int64_t)slot);
+        const int64_t cap = std::min<int64_t>(n, per);
+        int s2 = ensure_scratch(c, slot * (size_t)cap + (size_t)(cap + 8 + 1) / 2);
+        if (s2) return s2;
+        // 8 per-XCD queue counters, then one row counter per root
+        uint32_t* ctrs = reinterpret_cast<uint32_t*>(c->d_scr + slot * (size_t)cap);
+        // blocks per root: R-MAT 24 P14, 1 / 2 / 4 / 8 / 16 / 32 blocks: tier
+        // (five launch pairs) 1,734 / 1,444 / 1,193 / 1,075 / 1,068 / 1,071 ms
+        // for the query (the fused kernel 1,650)
+        const int rep = 8;
+        for (int64_t b = 0; b < n; b += cap) {
+          jk::CliqueArgs s = a;
+          s.ids = a.ids + b;
+          s.ntasks = std::min<int64_t>(cap, n - b);
+          s.gscr = c->d_scr;
+          s.rowctr = ctrs + 8;
+          s.rep = rep;
+          hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((s.ntasks + 8 + 255) / 256)), dim3(256), 0, st, ctrs,
+                             (int)(s.ntasks + 8), 0u);
+          for (int part = 0; part < 2; ++part) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, a.qctr, 1, 0u);
+            // resident blocks per CU: build 2 (16 waves, 8 per SIMD); count 1
+            // for 5-cliques (the 2x2 tiles' registers), 3 otherwise
+            const int per_cu = part == 0 ? 2 : k == 5 ? 1 : 3;
+            const int sg = (int)std::min<int64_t>((int64_t)per_cu * c->num_cu, part == 0 ? s.ntasks : s.ntasks * rep);
+            prof_begin(c, st, part == 0 ? "clique_glb2048_build" : kTierName[tier]);
+            hipError_t e = jk::launch_clique_glb_split(k, part == 0, s, sg, st);
+            prof_end(c, st);
+            if (e != hipSuccess) return JGPM_E_HIP;
+          }
+        }
+        continue;
+      }
       size_t sw = jk::tier_scratch_words(tier);
       if (sw) {
         grid = (int)std::min<int64_t>(grid, glb_tier_grid(c));

@@ -30,6 +30,9 @@ struct CliqueArgs {
   uint32_t* qctr;        // work-queue counter (zero before launch)
   uint64_t* gscr;        // global row scratch (TIER_GLB32)
   uint64_t* out;         // uint64 accumulator
+  uint32_t* rowctr;      // split 2048-wide count: one row counter per task, 8 queue
+                         // counters before them (all zero before)
+  int32_t rep;           // split 2048-wide count: blocks per task
 };
 
 // Query descriptor as the persistent kernel sees it.
@@ -51,6 +54,11 @@ hipError_t launch_clique(int k, bool bytes, int tier, const CliqueArgs& a, int g
 // a.gscr slots of split_slot_words() words per task), then count kernel.
 hipError_t launch_clique_split(int k, bool build, const CliqueArgs& a, int grid, hipStream_t st);
 size_t split_slot_words();
+// Split form of TIER_GLB32 for k >= 5: build kernel (16 waves per root, rows
+// to a.gscr slots of glb_slot_words() words per task), then count kernel
+// (a.rep blocks per task share the root's rows through a.rowctr[task]).
+hipError_t launch_clique_glb_split(int k, bool build, const CliqueArgs& a, int grid, hipStream_t st);
+size_t glb_slot_words();
 
 // Symmetric-graph patterns P1..P12: one wave per directed edge of [e0, e0+ne).
 struct SymArgs {
