@@ -209,7 +209,9 @@ int enqueue_sym(jgpm_graph_s* g, int p, int64_t rb, int64_t re, int slot, hipStr
   static const char* const kSymName[13] = {"", "sym_p1", "sym_p2", "sym_p3", "sym_p4", "sym_p5", "sym_p6",
                                            "sym_p7", "sym_p8", "sym_p9", "sym_p10", "sym_p11", "sym_p12"};
   prof_begin(c, st, kSymName[p]);
-  hipError_t e = jk::launch_sym(p, a, grid, st);
+  // a whole-graph P3 counts each triangle once instead of per directed edge
+  hipError_t e = (p == 3 && rb == 0 && re == g->sym.nv) ? jk::launch_p3_triangles(a, grid, st)
+                                                        : jk::launch_sym(p, a, grid, st);
   prof_end(c, st);
   if (e != hipSuccess) return JGPM_E_HIP;
   if (heavy) {

@@ -89,6 +89,9 @@ struct SymArgs {
   int32_t heavy_mid;
 };
 hipError_t launch_sym(int p, const SymArgs& a, int grid, hipStream_t st);
+// Whole-graph P3 from one intersection per undirected edge (each triangle
+// once, its three oriented terms); needs tri and nsum, e0 = 0, ne = all edges.
+hipError_t launch_p3_triangles(const SymArgs& a, int grid, hipStream_t st);
 // The heavy edges of P5, P7, P9, P11 and P12, one block per edge, popped
 // from list `shape` (0, 1, 2 as above) that launch_sym filled; shape 2 keeps
 // S in the block's slice of a.scr (a.scr_stride ints, grid slices).

@@ -111,6 +111,79 @@ __global__ void __launch_bounds__(64 * WPB) sym_kernel(SymArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Whole-graph P3 over triangles.  The per-edge value above is a sum over
+// (v0, v1, v2) with v0 v1 v2 a triangle and v2 > v1 of
+//   X(v0, v1) - (d(v2) - 2) - (t(v0, v2) - 1),  X = D0 - d1 - d0 + 1 - t(v0, v1),
+// and a triangle a < b < c meets v2 > v1 in exactly three orientations:
+// (a, b, c), (b, a, c) and (c, a, b).  So one intersection per undirected
+// edge (a, b), a < b, restricted to common neighbours c > b, finds every
+// triangle once, and t(a, c), t(b, c) come from c's positions in N(a) and
+// N(b) through the triangle array: half the intersections of the per-edge
+// form, each over the members above b only.  Every root's share is in the
+// total, so only a whole-graph query runs this way.
+__global__ void __launch_bounds__(64 * WPB) p3_tri_kernel(SymArgs a) {
+  __shared__ int32_t s_list[WPB][NLIST][CAP];
+  __shared__ int32_t s_stage[WPB][STAGE];
+  __shared__ int64_t s_part[WPB];
+  const int wv = threadIdx.x >> 6;
+  const int ln = lane();
+  Wave w{s_stage[wv]};
+  int32_t* glist = a.scr + ((int64_t)blockIdx.x * WPB + wv) * NLIST * a.scr_stride;
+  int64_t acc = 0;  // the lane's terms
+  for (;;) {
+    int64_t t = 0;
+    if (ln == 0) t = (int64_t)atomicAdd((unsigned long long*)a.qctr, (unsigned long long)a.chunk);
+    const int64_t base = __shfl(t, 0);
+    if (base >= a.ne) break;
+    const int64_t end = min(a.ne, base + (int64_t)a.chunk);
+    for (int64_t e = base; e < end; ++e) {
+      const int64_t eid = a.e0 + e;
+      const int32_t va = a.src[eid];
+      const int32_t vb = a.ci[eid];
+      if (vb <= va) continue;
+      const int64_t ba = a.rp[va], bb = a.rp[vb];
+      const int da = (int)(a.rp[va + 1] - ba), db = (int)(a.rp[vb + 1] - bb);
+      const int32_t* Na = a.ci + ba;
+      const int32_t* Nb = a.ci + bb;
+      const int oa = w_lb(Na, da, vb + 1), ob = w_lb(Nb, db, vb + 1);  // members above b
+      const int ma = da - oa, mb = db - ob;
+      if (ma == 0 || mb == 0) continue;
+      const bool small = min(ma, mb) <= CAP;
+      int32_t* S = small ? s_list[wv][0] : glist;
+      int32_t* pa = small ? s_list[wv][1] : glist + a.scr_stride;
+      int32_t* pb = small ? s_list[wv][2] : glist + 2 * a.scr_stride;
+      const int n = w_imat(w, Na + oa, ma, Nb + ob, mb, INT32_MAX, S, pa, pb);
+      const int64_t tab = a.tri[eid];
+      const int64_t xab = a.nsum[va] - db - da + 1 - tab;
+      const int64_t xba = a.nsum[vb] - da - db + 1 - tab;
+      for (int k = ln; k < n; k += 64) {
+        const int32_t c = S[k];
+        const int64_t tac = a.tri[ba + oa + pa[k]];
+        const int64_t tbc = a.tri[bb + ob + pb[k]];
+        const int64_t dc = a.rp[c + 1] - a.rp[c];
+        acc += xab - (dc - 2) - (tac - 1);                                // (a, b, c)
+        acc += xba - (dc - 2) - (tbc - 1);                                // (b, a, c)
+        acc += (a.nsum[c] - da - dc + 1 - tac) - (db - 2) - (tbc - 1);   // (c, a, b)
+      }
+      __syncwarp();  // the lists are rewritten by the next edge
+    }
+  }
+  acc = wsum64(acc);
+  if (ln == 0) s_part[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int k = 0; k < WPB; ++k) s += s_part[k];
+    if (s) atomicAdd((unsigned long long*)a.out, (unsigned long long)s);
+  }
+}
+
+hipError_t launch_p3_triangles(const SymArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(p3_tri_kernel, dim3(grid), dim3(64 * WPB), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Heavy edges of P5, P7, P9, P11 and P12.  Their per-edge value sums terms
 // of q(v2) = |S ∩ N(v2)| over the members v2 of S = N(v0) ∩ N(v1), and on
 // skewed graphs nearly all the work sits in the few edges with a large S
