@@ -130,7 +130,10 @@ static int64_t footprint(jgpm_graph_s* g, int p, const jgpm_ctx_s* c, int grid) 
     if (!g->d_nsum && p == JGPM_P3) b += 8 * (int64_t)g->sym.nv;
     b += (int64_t)grid * 4 * 5 * 4 * std::max(1, g->sym_mmin);  // list scratch of the batch kernel
   } else if (g->dag.max_deg > jk::tier_max_deg(jk::TIER_LDS16)) {
-    b += (int64_t)4 * c->num_cu * (int64_t)jk::tier_scratch_words(jk::TIER_GLB32) * 8;
+    // the 2048-wide tier's row scratch: slices of the fused kernel, or for
+    // 5- to 7-cliques the 512 KiB row slots of the split one (<= 1 GiB)
+    const int64_t fused = (int64_t)4 * c->num_cu * (int64_t)jk::tier_scratch_words(jk::TIER_GLB32) * 8;
+    b += jgpm::clique_k(p) >= 5 ? std::max<int64_t>(fused, (int64_t)1 << 30) : fused;
   }
   (void)c;
   return b;
