@@ -662,9 +662,19 @@ __global__ void __launch_bounds__(256) k_edge_tri(const int64_t* rp, const int32
     if (base >= ne) break;
     const int64_t end = min(ne, base + 64);
     for (int64_t e = base; e < end; ++e) {
+      // |N(u) ∩ N(v)| is the same for both directions: intersect once, from
+      // the smaller end, and write it to (v, u) too (found by a wave-wide
+      // lower bound in N(v))
       const int32_t u = src[e], v = ci[e];
-      int64_t c = w_icount(w, ci + rp[u], (int)(rp[u + 1] - rp[u]), ci + rp[v], (int)(rp[v + 1] - rp[v]), INT32_MAX);
-      if (ln == 0) tri[e] = (int32_t)c;
+      if (v < u) continue;
+      const int64_t bv = rp[v];
+      const int dv = (int)(rp[v + 1] - bv);
+      int64_t c = w_icount(w, ci + rp[u], (int)(rp[u + 1] - rp[u]), ci + bv, dv, INT32_MAX);
+      const int pr = w_lb(ci + bv, dv, u);
+      if (ln == 0) {
+        tri[e] = (int32_t)c;
+        tri[bv + pr] = (int32_t)c;
+      }
     }
   }
 }
