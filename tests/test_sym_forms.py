@@ -4,7 +4,9 @@ the oracle (the restated reference nests):
 * whole-graph P3 as a sum over triangles a < b < c of three oriented terms
   (a, b, c), (b, a, c), (c, a, b), the form of the triangle kernel
   (p3_tri_kernel), with a planted mistake (the orientation (a, c, b) in place
-  of (c, a, b)) that must change the total on the same graphs.
+  of (c, a, b)) that must change the total on the same graphs;
+* P5 per edge as (|S| - 2) |E(S)|, each edge of S counted once from its
+  smaller end, the form of the heavy-edge kernel (sym_heavy_kernel<5>).
 """
 import numpy as np
 import pytest
@@ -46,3 +48,33 @@ def test_p3_triangle_form_equals_oracle(scale, seed):
     want = O.count(g.rowptr, g.col, J.PATTERNS["P3"])
     assert p3_by_triangles(g) == want
     assert p3_by_triangles(g, plant=True) != want
+
+
+def p5_by_edges_of_s(g, plant=False):
+    """P5 as the heavy-edge kernel computes it: per edge (v0, v1), v1 < v0,
+    (|S| - 2) |E(S)| with S = N(v0) ∩ N(v1), each edge of S counted once from
+    its smaller end (the planted mistake counts it from both ends)."""
+    rp = np.asarray(g.rowptr)
+    ci = np.asarray(g.col)
+    nv = len(rp) - 1
+    nbr = [set(ci[rp[v]:rp[v + 1]].tolist()) for v in range(nv)]
+    tot = 0
+    for v0 in range(nv):
+        for v1 in nbr[v0]:
+            if v1 >= v0:
+                continue
+            s = nbr[v0] & nbr[v1]
+            n = len(s)
+            if n < 3:
+                continue
+            e = sum(1 for v2 in s for x in nbr[v2] if x in s and (plant or x > v2))
+            tot += (n - 2) * e
+    return tot
+
+
+@pytest.mark.parametrize("scale,seed", [(7, 2), (9, 4)])
+def test_p5_edges_of_s_form_equals_oracle(scale, seed):
+    g = J.rmat(scale, 8, seed=seed)
+    want = O.count(g.rowptr, g.col, J.PATTERNS["P5"])
+    assert p5_by_edges_of_s(g) == want
+    assert p5_by_edges_of_s(g, plant=True) != want
